@@ -1,0 +1,228 @@
+#!/usr/bin/env python3
+"""Benchmark: batch-SOM epoch throughput on one MI355X.
+
+Workload (one "step" = one full epoch over the dataset resident in HBM): 64x64 hexagonal map
+(4,096 neurons), dim 16, 12,582,912 samples (one per HEALPix nside=1024 pixel) of synthetic
+systematics (8 lognormal, 6 Gaussian, sin/cos of the pixel-centre colatitude, standardised),
+sigma0 = 32 on the reference's 20-epoch asymptotic schedule, init_rand(seed=0). An epoch is the
+reference's whole epoch: BMU search, per-BMU accumulation, neighbourhood update and division, plus
+that epoch's quantization/topographic errors (fused into the next BMU pass).
+
+value = samples/s of the BF16 + exact-rerank path; the plain FP32 path is measured on the same
+workload and reported beside it with the speed-up. Prints ONE JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "SOM epoch samples/s 64x64 D16, bf16+rerank vs fp32; BMU serve p50/p99 ms"
+PEAK_BF16_TFLOPS = 2500.0
+PEAK_FP32_TFLOPS = 157.3
+PEAK_HBM_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=18)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--nside", type=int, default=1024)
+    ap.add_argument("--map", type=int, default=64)
+    ap.add_argument("--epochs-schedule", type=int, default=20)
+    ap.add_argument("--arithmetic", default="ref32")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--precisions", default="fp32,bf16_rerank")
+    return ap.parse_args()
+
+
+def run_epochs(ctx, model_cls, acc_cls, dev, x_dev, n, W0, topo, m, prec, arith, steps, warmup, E, sigma0):
+    from decontamination_amd import _lib
+    pc, ac = _lib.PRECISIONS[prec], _lib.ARITHMETICS[arith]
+    mn, dim = W0.shape
+    model = model_cls(ctx, m, m, dim, "hexagonal", topo)
+    model.set_weights(W0)
+    acc = acc_cls(model)
+    total = warmup + steps
+    errs = dev.zeros(ctx, (total + 1, 3), np.float64)
+    hist = dev.empty(ctx, (E, mn, dim), np.float32)
+
+    def epoch(e):
+        acc.reset()
+        acc.epoch_accumulate(x_dev.ptr, None, n, pc, ac, errs.view_ptr((e - 1) * 3) if e > 0 else None)
+        acc.epoch_update(e % E, E, sigma0, hist.view_ptr((e % E) * mn * dim))
+
+    for e in range(warmup):
+        epoch(e)
+    ctx.synchronize()
+    ctx.set_timing(True)
+    t0 = time.perf_counter()
+    for e in range(warmup, total):
+        epoch(e)
+    ctx.synchronize()
+    t1 = time.perf_counter()
+    timing = ctx.timing()
+    ctx.set_timing(False)
+    e3 = errs.copy_to_host()
+    W = model.get_weights()
+    errs.free()
+    hist.free()
+    acc.close()
+    return {"seconds": t1 - t0, "timing": timing, "errors": e3, "W": W, "model": model}
+
+
+def roofline(timing, kernel, flops_per_launch, peak_tflops):
+    if kernel not in timing:
+        return None
+    ms, launches = timing[kernel]
+    avg_ms = ms / launches
+    achieved = flops_per_launch / (avg_ms * 1e-3) / 1e12
+    return {"bound": "mfma", "achieved": round(achieved, 3), "peak": peak_tflops, "unit": "TFLOP/s",
+            "frac": round(achieved / peak_tflops, 5), "traffic": None, "kernel": kernel,
+            "avg_ms": round(avg_ms, 4), "launches": launches}
+
+
+def cpu_baseline(X, W0, topo, sigma0, E, target_s=12.0):
+    """The oracle's parallel restatement of the reference CPU epoch (prange + FP32 atomics + errors
+    pass) on a prefix of the same workload; extrapolated per sample."""
+    import ctypes
+    import oracle
+    from oracle import ptr
+    L = oracle.lib()
+    mn, dim = W0.shape
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+
+    def run(p):
+        xs = np.ascontiguousarray(X[:p])
+        num = np.zeros((mn, dim), np.float32)
+        den = np.zeros(mn, np.float32)
+        err = np.zeros(3, np.float64)
+        t0 = time.perf_counter()
+        L.orc_train_epoch_accumulate_omp(ptr(num), ptr(den), ptr(W0), ptr(topo), ptr(xs), None, p, 0, E,
+                                         np.float32(sigma0), mn, dim)
+        Wn = np.empty((mn, dim), np.float32)
+        L.orc_divide(0, ptr(num), ptr(den), mn, dim, ptr(Wn))
+        L.orc_compute_errors_omp(ptr(Wn), ptr(topo), mn, dim, ptr(xs), None, p, 1.0, ptr(err))
+        return time.perf_counter() - t0
+
+    p = 1024
+    t = run(p)
+    p2 = int(min(len(X), max(p, p * target_s / max(t, 1e-3))))
+    p2 = max(1024, min(p2, 1 << 17))
+    t2 = run(p2)
+    return {"value": round(p2 / t2, 2), "unit": "samples/s", "cores": threads, "kind": "port",
+            "sample": f"first {p2} samples of the same 64x64/D16 workload, one epoch (reference loop nest: "
+                      f"parallel over samples, FP32 atomic scatter to all {mn} neurons, plus the errors pass), "
+                      f"{t2:.2f} s; extrapolated per sample"}
+
+
+def accuracy(ctx, res, X, topo, m, prec, arith, n_sub=65536):
+    """BMU disagreement of the GPU path against the FP64 oracle on a subsample, at the run's final weights."""
+    import decontamination_amd as dam
+    from oracle import som_ref as R
+    rng = np.random.default_rng(123)
+    idx = np.sort(rng.choice(len(X), size=min(n_sub, len(X)), replace=False))
+    Xs = np.ascontiguousarray(X[idx])
+    W = res["W"]
+    som = dam.SOM_Batch(m, m, W.shape[1], precision=prec, arithmetic=arith)
+    som._weights[...] = W
+    got = som.get_winners(Xs)
+    c1, c2, d1 = R.top2_fp64_np(W, Xs)
+    dis = float(np.mean(got != c1))
+    return {"subsample": len(idx), "bmu_disagreement_vs_fp64": dis, "bmu_agreement_vs_fp64": 1.0 - dis}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU; run with --gpus 1 (multi-GPU sharding is "
+                                   "described in DESIGN.md, not run)"}), flush=True)
+        sys.exit(2)
+    import decontamination_amd as dam
+    from decontamination_amd import synth
+    from decontamination_amd.runtime import Accumulator, DeviceArray, Model, get_context
+
+    m = args.map
+    E = args.epochs_schedule
+    sigma0 = m / 2.0
+    t = time.perf_counter()
+    X = synth.systematics_dataset(args.nside, 8, 6, seed=0)
+    N, dim = X.shape
+    log(f"data {N}x{dim} generated in {time.perf_counter() - t:.1f}s")
+    som = dam.SOM_Batch(m, m, dim)
+    som.init_rand(seed=0)
+    W0 = som.weights.copy()
+    topo = som._topography
+
+    ctx = get_context(0)
+    t = time.perf_counter()
+    x_dev = DeviceArray.from_host(ctx, X)
+    ctx.synchronize()
+    h2d_s = time.perf_counter() - t
+    log(f"uploaded {X.nbytes / 1e9:.2f} GB in {h2d_s:.2f}s")
+
+    results = {}
+    for prec in args.precisions.split(","):
+        log(f"timing {prec}: warmup {args.warmup}, steps {args.steps}")
+        r = run_epochs(ctx, Model, Accumulator, DeviceArray, x_dev, N, W0, topo, m, prec, args.arithmetic,
+                       args.steps, args.warmup, E, sigma0)
+        r["samples_per_s"] = N * args.steps / r["seconds"]
+        r["ms_per_step"] = 1e3 * r["seconds"] / args.steps
+        log(f"{prec}: {r['ms_per_step']:.3f} ms/epoch, {r['samples_per_s'] / 1e9:.3f} G samples/s; "
+            f"kernels {json.dumps({k: [round(v[0] / v[1], 4), v[1]] for k, v in r['timing'].items()})}")
+        results[prec] = r
+
+    flops = 2.0 * N * m * m * dim
+    head = results.get("bf16_rerank") or next(iter(results.values()))
+    head_prec = "bf16_rerank" if "bf16_rerank" in results else next(iter(results))
+    out = {
+        "metric": METRIC,
+        "value": round(head["samples_per_s"], 1),
+        "unit": "samples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(head["ms_per_step"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16" if head_prec == "bf16_rerank" else "fp32",
+        "data": "synthetic",
+        "config": {"workload": f"batch SOM epoch, {m}x{m} hexagonal, dim {dim}, N={N} (HEALPix nside "
+                               f"{args.nside}), sigma0={sigma0:g} on the {E}-epoch schedule, precision "
+                               f"{head_prec}, rerank arithmetic {args.arithmetic}",
+                   "map": f"{m}x{m}", "dim": dim, "samples": N, "precision": head_prec},
+        "roofline": roofline(head["timing"], "bmu_bf16" if head_prec == "bf16_rerank" else "bmu_fp32", flops,
+                             PEAK_BF16_TFLOPS if head_prec == "bf16_rerank" else PEAK_FP32_TFLOPS),
+        "h2d_upload_s": round(h2d_s, 4),
+    }
+    if "fp32" in results and "bf16_rerank" in results:
+        f = results["fp32"]
+        out["fp32"] = {"value": round(f["samples_per_s"], 1), "ms_per_step": round(f["ms_per_step"], 4),
+                       "roofline": roofline(f["timing"], "bmu_fp32", flops, PEAK_FP32_TFLOPS)}
+        out["speedup_bf16_rerank_vs_fp32"] = round(head["samples_per_s"] / f["samples_per_s"], 3)
+    out["kernels_ms_per_step"] = {k: round(v[0] / args.steps, 4) for k, v in head["timing"].items()}
+    if not args.no_accuracy:
+        acc = {}
+        for prec, r in results.items():
+            acc[prec] = accuracy(ctx, r, X, topo, m, prec, args.arithmetic)
+        out["accuracy"] = acc
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(X, W0, topo, sigma0, E)
+    out["serving"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,643 @@
+// extern "C" entry points of libsomhip (declared in include/somhip.h).
+#include "common.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <numeric>
+#include <string>
+#include <vector>
+
+// launchers from k_train.hip
+int somhip_launch_absmax(hipStream_t st, const float *x, int dim, int dpad, const double *rho, int rho_f32, int64_t n,
+                         unsigned int *bounds, int num_cus);
+int somhip_launch_shifts(hipStream_t st, const unsigned int *bounds, int dim, int has_x, int has_rho, int64_t n,
+                         int *shifts);
+int somhip_scatter_nwg(int64_t n, int mn, int nv, int num_cus, size_t table_budget);
+int somhip_launch_scatter(hipStream_t st, const int32_t *bmu, const float *x, int dim, const double *rho, int rho_f32,
+                          int64_t n, const int *shifts, int mn, int nv, int nwg, unsigned long long *partial,
+                          double *acc, size_t lds_limit);
+int somhip_update_nsplit(int mn);
+int somhip_launch_update(hipStream_t st, const double *S, const float *topo, int mn, int dim, double sigma,
+                         double *part, int nsplit, float *w, float *hist);
+int somhip_launch_divide(hipStream_t st, const double *acc, int mn, int dim, float *w, float *hist);
+int somhip_iter_nsplit(int64_t n, int mn);
+int somhip_launch_iter(hipStream_t st, const int32_t *bmu, const float *x, int dim, int dpad, const double *rho,
+                       int64_t n, int64_t cur, int64_t nvec, float sigma0, const float *topo, int mn, double *part,
+                       int nsplit, double *acc);
+int somhip_errors_nparts(int64_t n);
+int somhip_launch_errors(hipStream_t st, const int32_t *c1, const int32_t *c2, const double *d1, const double *rho,
+                         int64_t n, const float *topo, double penalty, double *part, int nparts, double *errors);
+
+namespace {
+
+thread_local std::string g_last_error;
+
+constexpr size_t SCATTER_TABLE_BUDGET = (size_t)192 << 20;  // bytes of per-workgroup partial tables
+
+inline int check_model(const somhip_model *M)
+{
+    if (!M || !M->ctx) {
+        somhip_set_error("null model");
+        return SOMHIP_ERR_ARG;
+    }
+    return SOMHIP_OK;
+}
+
+inline int check_prec(int precision, int arithmetic)
+{
+    if (precision < SOMHIP_PREC_EXACT || precision > SOMHIP_PREC_BF16_RERANK) {
+        somhip_set_error("invalid precision " + std::to_string(precision));
+        return SOMHIP_ERR_ARG;
+    }
+    if (arithmetic != SOMHIP_RERANK_REF32 && arithmetic != SOMHIP_RERANK_FP64) {
+        somhip_set_error("invalid arithmetic " + std::to_string(arithmetic));
+        return SOMHIP_ERR_ARG;
+    }
+    return SOMHIP_OK;
+}
+
+// Choose a slot permutation for the BF16 path: slot p -> neuron (p*S) mod mn, with S coprime to mn
+// and no multiple k*S (k < 32) landing on a grid neighbour offset, so that a neuron and its grid
+// neighbours never share a 32-slot block. Padding slots map to -1.
+std::vector<int> make_perm(int m, int n, int nblk)
+{
+    const int mn = m * n;
+    std::vector<int> perm((size_t)nblk * 32, -1);
+    int S = 1;
+    if (mn > 64) {
+        auto bad = [&](int s) {
+            if (std::gcd(s, mn) != 1) return true;
+            for (int k = 1; k < 32; k++) {
+                long long o = ((long long)k * s) % mn;
+                long long d = std::min(o, (long long)mn - o);
+                if (d <= 1 || std::llabs(d - n) <= 1) return true;
+            }
+            return false;
+        };
+        const int start = std::max(2, (int)(0.6180339887 * mn));
+        S = 0;
+        for (int s = start; s < mn && !S; s++)
+            if (!bad(s)) S = s;
+        for (int s = start - 1; s > 1 && !S; s--)
+            if (!bad(s)) S = s;
+        if (!S) S = 1;
+    }
+    for (int p = 0; p < mn; p++) perm[p] = (int)(((long long)p * S) % mn);
+    return perm;
+}
+
+}  // namespace
+
+void somhip_set_error(const std::string &msg) { g_last_error = msg; }
+
+static hipEvent_t timer_event(somhip_ctx *C)
+{
+    if (!C->timer.pool.empty()) {
+        hipEvent_t e = C->timer.pool.back();
+        C->timer.pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+TimerScope::TimerScope(somhip_ctx *c, int i) : C(c), id(i)
+{
+    if (!C || !C->timer.enabled) return;
+    a = timer_event(C);
+    b = timer_event(C);
+    if (a) (void)hipEventRecord(a, C->stream);
+}
+
+TimerScope::~TimerScope()
+{
+    if (!C || !C->timer.enabled || !a || !b) return;
+    (void)hipEventRecord(b, C->stream);
+    C->timer.pending.push_back({id, a, b});
+}
+
+int somhip_timer_collect(somhip_ctx *C)
+{
+    for (auto &p : C->timer.pending) {
+        SOM_HIP_TRY(hipEventSynchronize(p.b));
+        float ms = 0.f;
+        SOM_HIP_TRY(hipEventElapsedTime(&ms, p.a, p.b));
+        C->timer.total_ms[p.id] += ms;
+        C->timer.launches[p.id] += 1;
+        C->timer.pool.push_back(p.a);
+        C->timer.pool.push_back(p.b);
+    }
+    C->timer.pending.clear();
+    return SOMHIP_OK;
+}
+
+struct somhip_accum {
+    somhip_model *model = nullptr;
+    int mode = 0;  // 0 empty, 1 epoch sums, 2 vector-count numerator/denominator
+    double *acc = nullptr;  // (mn, dim+1) FP64
+    Workspace ws;
+    DevBuf upd;
+};
+
+extern "C" {
+
+const char *somhip_last_error(void) { return g_last_error.c_str(); }
+
+const char *somhip_version(void) { return "somhip 0.1.0 (gfx950)"; }
+
+int somhip_device_count(int *count)
+{
+    if (!count) return SOMHIP_ERR_ARG;
+    int c = 0;
+    hipError_t e = hipGetDeviceCount(&c);
+    *count = (e == hipSuccess) ? c : 0;
+    return SOMHIP_OK;
+}
+
+int somhip_ctx_create(int device, somhip_ctx **out)
+{
+    if (!out) {
+        somhip_set_error("null output pointer");
+        return SOMHIP_ERR_ARG;
+    }
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) {
+        somhip_set_error("no HIP device visible");
+        return SOMHIP_ERR_NODEVICE;
+    }
+    if (device < 0 || device >= count) {
+        somhip_set_error("device index out of range");
+        return SOMHIP_ERR_ARG;
+    }
+    SOM_HIP_TRY(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    SOM_HIP_TRY(hipGetDeviceProperties(&prop, device));
+    if (std::string(prop.gcnArchName).rfind("gfx950", 0) != 0) {
+        somhip_set_error(std::string("libsomhip is built for gfx950; device is ") + prop.gcnArchName);
+        return SOMHIP_ERR_NODEVICE;
+    }
+    somhip_ctx *C = new somhip_ctx();
+    C->device = device;
+    C->num_cus = prop.multiProcessorCount;
+    C->lds_per_block = prop.sharedMemPerBlock;
+    hipError_t e = hipStreamCreateWithFlags(&C->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete C;
+        somhip_set_error(std::string("hipStreamCreate: ") + hipGetErrorString(e));
+        return SOMHIP_ERR_HIP;
+    }
+    C->own_stream = C->stream;
+    *out = C;
+    return SOMHIP_OK;
+}
+
+int somhip_ctx_destroy(somhip_ctx *C)
+{
+    if (!C) return SOMHIP_OK;
+    (void)hipSetDevice(C->device);
+    (void)hipStreamSynchronize(C->stream);
+    C->ws.release();
+    for (auto &p : C->timer.pending) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
+    for (auto e : C->timer.pool) (void)hipEventDestroy(e);
+    if (C->scratch) (void)hipFree(C->scratch);
+    if (C->own_stream) (void)hipStreamDestroy(C->own_stream);
+    delete C;
+    return SOMHIP_OK;
+}
+
+int somhip_ctx_synchronize(somhip_ctx *C)
+{
+    if (!C) return SOMHIP_ERR_ARG;
+    SOM_HIP_TRY(hipStreamSynchronize(C->stream));
+    return SOMHIP_OK;
+}
+
+int somhip_ctx_set_stream(somhip_ctx *C, void *s)
+{
+    if (!C) return SOMHIP_ERR_ARG;
+    SOM_HIP_TRY(hipStreamSynchronize(C->stream));
+    C->stream = s ? (hipStream_t)s : C->own_stream;
+    return SOMHIP_OK;
+}
+
+int somhip_ctx_get_stream(somhip_ctx *C, void **s)
+{
+    if (!C || !s) return SOMHIP_ERR_ARG;
+    *s = (void *)C->stream;
+    return SOMHIP_OK;
+}
+
+int somhip_ctx_set_timing(somhip_ctx *C, int enable)
+{
+    if (!C) return SOMHIP_ERR_ARG;
+    int rc = somhip_timer_collect(C);
+    if (rc) return rc;
+    C->timer.enabled = enable != 0;
+    for (int i = 0; i < 16; i++) {
+        C->timer.total_ms[i] = 0.0;
+        C->timer.launches[i] = 0;
+    }
+    return SOMHIP_OK;
+}
+
+int somhip_ctx_get_timing(somhip_ctx *C, int kernel, double *total_ms, int64_t *launches)
+{
+    if (!C || kernel < 0 || kernel >= TK_COUNT || !total_ms || !launches) return SOMHIP_ERR_ARG;
+    int rc = somhip_timer_collect(C);
+    if (rc) return rc;
+    *total_ms = C->timer.total_ms[kernel];
+    *launches = C->timer.launches[kernel];
+    return SOMHIP_OK;
+}
+
+int somhip_malloc(somhip_ctx *C, size_t bytes, void **dptr)
+{
+    if (!C || !dptr) return SOMHIP_ERR_ARG;
+    SOM_HIP_TRY(hipSetDevice(C->device));
+    SOM_HIP_TRY(hipMalloc(dptr, bytes ? bytes : 1));
+    return SOMHIP_OK;
+}
+
+int somhip_free(somhip_ctx *C, void *dptr)
+{
+    if (!C) return SOMHIP_ERR_ARG;
+    if (!dptr) return SOMHIP_OK;
+    SOM_HIP_TRY(hipStreamSynchronize(C->stream));
+    SOM_HIP_TRY(hipFree(dptr));
+    return SOMHIP_OK;
+}
+
+int somhip_memcpy_h2d(somhip_ctx *C, void *dst, const void *src, size_t bytes)
+{
+    if (!C || (bytes && (!dst || !src))) return SOMHIP_ERR_ARG;
+    if (!bytes) return SOMHIP_OK;
+    SOM_HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, C->stream));
+    SOM_HIP_TRY(hipStreamSynchronize(C->stream));
+    return SOMHIP_OK;
+}
+
+int somhip_memcpy_d2h(somhip_ctx *C, void *dst, const void *src, size_t bytes)
+{
+    if (!C || (bytes && (!dst || !src))) return SOMHIP_ERR_ARG;
+    if (!bytes) return SOMHIP_OK;
+    SOM_HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, C->stream));
+    SOM_HIP_TRY(hipStreamSynchronize(C->stream));
+    return SOMHIP_OK;
+}
+
+int somhip_memcpy_d2d(somhip_ctx *C, void *dst, const void *src, size_t bytes)
+{
+    if (!C || (bytes && (!dst || !src))) return SOMHIP_ERR_ARG;
+    if (!bytes) return SOMHIP_OK;
+    SOM_HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, C->stream));
+    return SOMHIP_OK;
+}
+
+int somhip_memset(somhip_ctx *C, void *dptr, int value, size_t bytes)
+{
+    if (!C || (bytes && !dptr)) return SOMHIP_ERR_ARG;
+    if (!bytes) return SOMHIP_OK;
+    SOM_HIP_TRY(hipMemsetAsync(dptr, value, bytes, C->stream));
+    return SOMHIP_OK;
+}
+
+int somhip_model_create(somhip_ctx *C, int m, int n, int dim, int topology, const float *topo_host, somhip_model **out)
+{
+    if (!C || !out || !topo_host) {
+        somhip_set_error("null argument");
+        return SOMHIP_ERR_ARG;
+    }
+    *out = nullptr;
+    if (m <= 0 || n <= 0 || dim <= 0) {
+        somhip_set_error("m, n and dim must be positive");
+        return SOMHIP_ERR_ARG;
+    }
+    if (dim > 64) {
+        somhip_set_error("dim > 64 is not supported");
+        return SOMHIP_ERR_UNSUPPORTED;
+    }
+    if ((long long)m * n > (1 << 24)) {
+        somhip_set_error("too many neurons");
+        return SOMHIP_ERR_UNSUPPORTED;
+    }
+    if (topology != SOMHIP_TOPOLOGY_SQUARE && topology != SOMHIP_TOPOLOGY_HEXAGONAL) {
+        somhip_set_error("invalid topology");
+        return SOMHIP_ERR_ARG;
+    }
+    SOM_HIP_TRY(hipSetDevice(C->device));
+    somhip_model *M = new somhip_model();
+    M->ctx = C;
+    M->m = m; M->n = n; M->mn = m * n; M->dim = dim;
+    M->dpad = (dim + 15) / 16 * 16;
+    M->topology = topology;
+    M->nblk = (M->mn + 31) / 32;
+    const size_t slots = (size_t)M->nblk * 32;
+    auto fail = [&](hipError_t e, const char *what) {
+        somhip_set_error(std::string(what) + ": " + hipGetErrorString(e));
+        somhip_model_destroy(M);
+        return SOMHIP_ERR_HIP;
+    };
+    hipError_t e;
+#define SOM_ALLOC(ptr, bytes)                                  \
+    if ((e = hipMalloc((void **)&(ptr), (bytes))) != hipSuccess) \
+        return fail(e, "hipMalloc");
+    SOM_ALLOC(M->w, sizeof(float) * M->mn * dim);
+    SOM_ALLOC(M->topo, sizeof(float) * M->mn * 2);
+    SOM_ALLOC(M->centre, sizeof(float) * M->dpad);
+    SOM_ALLOC(M->wmax, sizeof(float) * M->dpad);
+    SOM_ALLOC(M->scal, sizeof(float) * 4);
+    SOM_ALLOC(M->norms_perm, sizeof(float) * slots);
+    SOM_ALLOC(M->norms_nat, sizeof(float) * slots);
+    SOM_ALLOC(M->frag_bf, sizeof(unsigned short) * slots * M->dpad * 2);
+    SOM_ALLOC(M->frag_f32, sizeof(float) * slots * M->dpad);
+    SOM_ALLOC(M->perm, sizeof(int) * slots);
+#undef SOM_ALLOC
+    std::vector<int> perm = make_perm(m, n, M->nblk);
+    if ((e = hipMemcpyAsync(M->perm, perm.data(), sizeof(int) * slots, hipMemcpyHostToDevice, C->stream)) != hipSuccess)
+        return fail(e, "hipMemcpy perm");
+    if ((e = hipMemcpyAsync(M->topo, topo_host, sizeof(float) * M->mn * 2, hipMemcpyHostToDevice, C->stream)) != hipSuccess)
+        return fail(e, "hipMemcpy topography");
+    if ((e = hipMemsetAsync(M->w, 0, sizeof(float) * M->mn * dim, C->stream)) != hipSuccess) return fail(e, "hipMemset");
+    if ((e = hipStreamSynchronize(C->stream)) != hipSuccess) return fail(e, "hipStreamSynchronize");
+    M->prepared = false;
+    *out = M;
+    return SOMHIP_OK;
+}
+
+int somhip_model_destroy(somhip_model *M)
+{
+    if (!M) return SOMHIP_OK;
+    if (M->ctx) (void)hipStreamSynchronize(M->ctx->stream);
+    void *ptrs[] = {M->w, M->topo, M->centre, M->wmax, M->scal, M->norms_perm, M->norms_nat, M->frag_bf, M->frag_f32, M->perm};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    delete M;
+    return SOMHIP_OK;
+}
+
+int somhip_model_set_weights(somhip_model *M, const float *w, int on_device)
+{
+    int rc = check_model(M);
+    if (rc) return rc;
+    if (!w) {
+        somhip_set_error("null weights");
+        return SOMHIP_ERR_ARG;
+    }
+    SOM_HIP_TRY(hipMemcpyAsync(M->w, w, sizeof(float) * M->mn * M->dim,
+                               on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, M->ctx->stream));
+    if (!on_device) SOM_HIP_TRY(hipStreamSynchronize(M->ctx->stream));
+    M->prepared = false;
+    return SOMHIP_OK;
+}
+
+int somhip_model_get_weights(somhip_model *M, float *w_host)
+{
+    int rc = check_model(M);
+    if (rc) return rc;
+    if (!w_host) return SOMHIP_ERR_ARG;
+    SOM_HIP_TRY(hipMemcpyAsync(w_host, M->w, sizeof(float) * M->mn * M->dim, hipMemcpyDeviceToHost, M->ctx->stream));
+    SOM_HIP_TRY(hipStreamSynchronize(M->ctx->stream));
+    return SOMHIP_OK;
+}
+
+int somhip_model_weights_ptr(somhip_model *M, float **dptr)
+{
+    int rc = check_model(M);
+    if (rc) return rc;
+    if (!dptr) return SOMHIP_ERR_ARG;
+    *dptr = M->w;
+    return SOMHIP_OK;
+}
+
+int somhip_find_bmus(somhip_model *M, const float *x, int64_t n, int precision, int arithmetic, int32_t *out)
+{
+    int rc = check_model(M);
+    if (rc) return rc;
+    if ((rc = check_prec(precision, arithmetic))) return rc;
+    if (n < 0 || (n > 0 && (!x || !out))) {
+        somhip_set_error("invalid x/out/n");
+        return SOMHIP_ERR_ARG;
+    }
+    return somhip_launch_bmu(M, x, n, precision, arithmetic, 0, 0, out, nullptr, nullptr);
+}
+
+int somhip_count_bmus(somhip_model *M, const float *x, const double *rho, int64_t n, int precision, int arithmetic,
+                      double *result)
+{
+    int rc = check_model(M);
+    if (rc) return rc;
+    if ((rc = check_prec(precision, arithmetic))) return rc;
+    if (n < 0 || (n > 0 && (!x || !result))) {
+        somhip_set_error("invalid x/result/n");
+        return SOMHIP_ERR_ARG;
+    }
+    if (n == 0) return SOMHIP_OK;
+    somhip_ctx *C = M->ctx;
+    Workspace &W = C->ws;
+    const int nwg = somhip_scatter_nwg(n, M->mn, 1, C->num_cus, SCATTER_TABLE_BUDGET);
+    if ((rc = W.c1.ensure(sizeof(int32_t) * n)) || (rc = W.bounds.ensure(256)) || (rc = W.shifts.ensure(256)) ||
+        (rc = W.table.ensure((size_t)nwg * M->mn * 8)))
+        return rc;
+    int32_t *c1 = (int32_t *)W.c1.p;
+    if ((rc = somhip_launch_bmu(M, x, n, precision, arithmetic, 0, 0, c1, nullptr, nullptr))) return rc;
+    {
+        TimerScope ts(C, TK_ABSMAX);
+        if ((rc = somhip_launch_absmax(C->stream, nullptr, 0, 16, rho, 0, rho ? n : 0, (unsigned int *)W.bounds.p,
+                                       C->num_cus)))
+            return rc;
+        if ((rc = somhip_launch_shifts(C->stream, (const unsigned int *)W.bounds.p, 0, 0, rho != nullptr, n,
+                                       (int *)W.shifts.p)))
+            return rc;
+    }
+    TimerScope ts(C, TK_SCATTER);
+    return somhip_launch_scatter(C->stream, c1, nullptr, 0, rho, 0, n, (const int *)W.shifts.p, M->mn, 1, nwg,
+                                 (unsigned long long *)W.table.p, result, C->lds_per_block);
+}
+
+int somhip_compute_errors(somhip_model *M, const float *x, const double *rho, int64_t n, int precision,
+                          int arithmetic, double *errors)
+{
+    int rc = check_model(M);
+    if (rc) return rc;
+    if ((rc = check_prec(precision, arithmetic))) return rc;
+    if (n < 0 || (n > 0 && (!x || !errors))) {
+        somhip_set_error("invalid x/errors/n");
+        return SOMHIP_ERR_ARG;
+    }
+    if (n == 0) return SOMHIP_OK;
+    somhip_ctx *C = M->ctx;
+    Workspace &W = C->ws;
+    const int nparts = somhip_errors_nparts(n);
+    if ((rc = W.c1.ensure(sizeof(int32_t) * n)) || (rc = W.c2.ensure(sizeof(int32_t) * n)) ||
+        (rc = W.d1.ensure(sizeof(double) * n)) || (rc = W.part.ensure(sizeof(double) * 3 * nparts)))
+        return rc;
+    int32_t *c1 = (int32_t *)W.c1.p, *c2 = (int32_t *)W.c2.p;
+    double *d1 = (double *)W.d1.p;
+    if ((rc = somhip_launch_bmu(M, x, n, precision, arithmetic, 1, -1, c1, c2, d1))) return rc;
+    const double penalty = M->topology == SOMHIP_TOPOLOGY_SQUARE ? 2.0 : 1.0;
+    TimerScope ts(C, TK_ERRORS);
+    return somhip_launch_errors(C->stream, c1, c2, d1, rho, n, M->topo, penalty, (double *)W.part.p, nparts, errors);
+}
+
+int somhip_accum_create(somhip_model *M, somhip_accum **out)
+{
+    int rc = check_model(M);
+    if (rc) return rc;
+    if (!out) return SOMHIP_ERR_ARG;
+    somhip_accum *A = new somhip_accum();
+    A->model = M;
+    hipError_t e = hipMalloc((void **)&A->acc, sizeof(double) * M->mn * (M->dim + 1));
+    if (e != hipSuccess) {
+        delete A;
+        somhip_set_error(std::string("hipMalloc: ") + hipGetErrorString(e));
+        return SOMHIP_ERR_HIP;
+    }
+    *out = A;
+    return somhip_accum_reset(A);
+}
+
+int somhip_accum_destroy(somhip_accum *A)
+{
+    if (!A) return SOMHIP_OK;
+    (void)hipStreamSynchronize(A->model->ctx->stream);
+    if (A->acc) (void)hipFree(A->acc);
+    A->ws.release();
+    A->upd.release();
+    delete A;
+    return SOMHIP_OK;
+}
+
+int somhip_accum_reset(somhip_accum *A)
+{
+    if (!A) return SOMHIP_ERR_ARG;
+    somhip_model *M = A->model;
+    SOM_HIP_TRY(hipMemsetAsync(A->acc, 0, sizeof(double) * M->mn * (M->dim + 1), M->ctx->stream));
+    A->mode = 0;
+    return SOMHIP_OK;
+}
+
+int somhip_batch_epoch_accumulate(somhip_accum *A, const float *x, const double *rho, int64_t n, int precision,
+                                  int arithmetic, double *errors)
+{
+    if (!A) return SOMHIP_ERR_ARG;
+    somhip_model *M = A->model;
+    int rc = check_prec(precision, arithmetic);
+    if (rc) return rc;
+    if (A->mode == 2) {
+        somhip_set_error("accumulator holds vector-count sums; reset it first");
+        return SOMHIP_ERR_STATE;
+    }
+    if (n < 0 || (n > 0 && !x)) {
+        somhip_set_error("invalid x/n");
+        return SOMHIP_ERR_ARG;
+    }
+    A->mode = 1;
+    if (n == 0) return SOMHIP_OK;
+    somhip_ctx *C = M->ctx;
+    const int nv = M->dim + 1;
+    const int nwg = somhip_scatter_nwg(n, M->mn, nv, C->num_cus, SCATTER_TABLE_BUDGET);
+    const int nparts = somhip_errors_nparts(n);
+    Workspace &W = A->ws;
+    if ((rc = W.c1.ensure(sizeof(int32_t) * n)) || (rc = W.c2.ensure(sizeof(int32_t) * n)) ||
+        (rc = W.d1.ensure(sizeof(double) * n)) || (rc = W.part.ensure(sizeof(double) * 3 * nparts)) ||
+        (rc = W.bounds.ensure(512)) || (rc = W.shifts.ensure(512)) ||
+        (rc = W.table.ensure((size_t)nwg * M->mn * nv * 8)))
+        return rc;
+    int32_t *c1 = (int32_t *)W.c1.p, *c2 = (int32_t *)W.c2.p;
+    double *d1 = (double *)W.d1.p;
+    const bool need_err = errors != nullptr;
+    if ((rc = somhip_launch_bmu(M, x, n, precision, arithmetic, need_err ? 1 : 0, -1, c1, need_err ? c2 : nullptr,
+                                need_err ? d1 : nullptr)))
+        return rc;
+    if (need_err) {
+        const double penalty = M->topology == SOMHIP_TOPOLOGY_SQUARE ? 2.0 : 1.0;
+        TimerScope ts(C, TK_ERRORS);
+        if ((rc = somhip_launch_errors(C->stream, c1, c2, d1, rho, n, M->topo, penalty, (double *)W.part.p, nparts,
+                                       errors)))
+            return rc;
+    }
+    {
+        TimerScope ts(C, TK_ABSMAX);
+        if ((rc = somhip_launch_absmax(C->stream, x, M->dim, M->dpad, rho, 1, n, (unsigned int *)W.bounds.p,
+                                       C->num_cus)))
+            return rc;
+        if ((rc = somhip_launch_shifts(C->stream, (const unsigned int *)W.bounds.p, M->dim, 1, rho != nullptr, n,
+                                       (int *)W.shifts.p)))
+            return rc;
+    }
+    TimerScope ts(C, TK_SCATTER);
+    return somhip_launch_scatter(C->stream, c1, x, M->dim, rho, 1, n, (const int *)W.shifts.p, M->mn, nv, nwg,
+                                 (unsigned long long *)W.table.p, A->acc, C->lds_per_block);
+}
+
+int somhip_batch_epoch_update(somhip_accum *A, int64_t cur_epoch, int64_t n_epochs, float sigma0, float *history_out)
+{
+    if (!A) return SOMHIP_ERR_ARG;
+    if (A->mode != 1) {
+        somhip_set_error("epoch update without epoch accumulation");
+        return SOMHIP_ERR_STATE;
+    }
+    if (n_epochs <= 0 || cur_epoch < 0) {
+        somhip_set_error("invalid epoch counters");
+        return SOMHIP_ERR_ARG;
+    }
+    somhip_model *M = A->model;
+    // sigma = sigma0 * asymptotic_decay(e, E) >= FLOAT_EPS (som_batch.py:107-110, algo/__init__.py:64-67)
+    double sigma = (double)sigma0 * (1.0 / (1.0 + 2.0 * (double)cur_epoch / (double)n_epochs));
+    if (sigma < SOM_FLOAT_EPS) sigma = SOM_FLOAT_EPS;
+    const int nsplit = somhip_update_nsplit(M->mn);
+    int rc = A->upd.ensure(sizeof(double) * (size_t)nsplit * M->mn * (M->dim + 1));
+    if (rc) return rc;
+    TimerScope ts(M->ctx, TK_UPDATE);
+    rc = somhip_launch_update(M->ctx->stream, A->acc, M->topo, M->mn, M->dim, sigma, (double *)A->upd.p, nsplit, M->w,
+                              history_out);
+    M->prepared = false;
+    return rc;
+}
+
+int somhip_batch_iter_accumulate(somhip_accum *A, const float *x, const double *rho, int64_t n, int64_t cur_vector,
+                                 int64_t n_vectors, float sigma0, int precision, int arithmetic)
+{
+    if (!A) return SOMHIP_ERR_ARG;
+    somhip_model *M = A->model;
+    int rc = check_prec(precision, arithmetic);
+    if (rc) return rc;
+    if (A->mode == 1) {
+        somhip_set_error("accumulator holds epoch sums; reset it first");
+        return SOMHIP_ERR_STATE;
+    }
+    if (n < 0 || (n > 0 && !x) || n_vectors <= 0 || cur_vector < 0) {
+        somhip_set_error("invalid x/n/vector counters");
+        return SOMHIP_ERR_ARG;
+    }
+    A->mode = 2;
+    if (n == 0) return SOMHIP_OK;
+    somhip_ctx *C = M->ctx;
+    Workspace &W = A->ws;
+    const int nsplit = somhip_iter_nsplit(n, M->mn);
+    if ((rc = W.c1.ensure(sizeof(int32_t) * n)) ||
+        (rc = A->upd.ensure(sizeof(double) * (size_t)nsplit * M->mn * (M->dim + 1))))
+        return rc;
+    int32_t *c1 = (int32_t *)W.c1.p;
+    if ((rc = somhip_launch_bmu(M, x, n, precision, arithmetic, 0, -1, c1, nullptr, nullptr))) return rc;
+    TimerScope ts(C, TK_ITER);
+    return somhip_launch_iter(C->stream, c1, x, M->dim, M->dpad, rho, n, cur_vector, n_vectors, sigma0, M->topo, M->mn,
+                              (double *)A->upd.p, nsplit, A->acc);
+}
+
+int somhip_batch_iter_update(somhip_accum *A, float *history_out)
+{
+    if (!A) return SOMHIP_ERR_ARG;
+    if (A->mode != 2) {
+        somhip_set_error("vector-count update without vector-count accumulation");
+        return SOMHIP_ERR_STATE;
+    }
+    somhip_model *M = A->model;
+    int rc = somhip_launch_divide(M->ctx->stream, A->acc, M->mn, M->dim, M->w, history_out);
+    M->prepared = false;
+    return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,204 @@
+// Shared device helpers and internal host structures for libsomhip (gfx950 / CDNA4 only).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <math.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/somhip.h"
+
+// ------------------------------------------------------------------------------------------------
+// Vector types for the MFMA builtins.
+// ------------------------------------------------------------------------------------------------
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
+
+#define SOM_FLOAT_EPS 1.0e-12
+
+// 32x32 MFMA accumulator layout (v_mfma_f32_32x32x{2f32,16_bf16}): lane l, register r holds
+// D[row][col] with col = l % 32 and row = 8*(r/4) + 4*(l/32) + (r%4).
+__device__ __forceinline__ int acc_row(int r, int lane) { return 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3); }
+
+// Round-to-nearest-even float -> bf16 bits (NaN kept quiet).
+__host__ __device__ __forceinline__ unsigned short f32_to_bf16_bits(float f)
+{
+    unsigned int u = __builtin_bit_cast(unsigned int, f);
+    if ((u & 0x7F800000u) == 0x7F800000u && (u & 0x007FFFFFu)) return (unsigned short)((u >> 16) | 0x40u);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
+}
+
+__host__ __device__ __forceinline__ float bf16_bits_to_f32(unsigned short b)
+{
+    return __builtin_bit_cast(float, (unsigned int)b << 16);
+}
+
+// The reference's square_distance_xpu (decontamination/algo/__init__.py:89-98): FP32 difference,
+// FP32 square (numba types float32 ** int as float32), FP64 sequential accumulation.
+// The explicit _rn intrinsics keep the compiler from contracting or reassociating.
+__device__ __forceinline__ double sqd_ref32(const float *a, const float *b, int dim)
+{
+    double r = 0.0;
+    for (int k = 0; k < dim; k++) {
+        float d = __fsub_rn(a[k], b[k]);
+        float s = __fmul_rn(d, d);
+        r = __dadd_rn(r, (double)s);
+    }
+    return r;
+}
+
+// FP64 direct difference (the FP64 oracle's arithmetic).
+__device__ __forceinline__ double sqd_fp64(const float *a, const float *b, int dim)
+{
+    double r = 0.0;
+    for (int k = 0; k < dim; k++) {
+        double d = __dsub_rn((double)a[k], (double)b[k]);
+        r = __dadd_rn(r, __dmul_rn(d, d));
+    }
+    return r;
+}
+
+template <int MODE>
+__device__ __forceinline__ double sqd_mode(const float *a, const float *b, int dim)
+{
+    if constexpr (MODE == SOMHIP_RERANK_FP64) return sqd_fp64(a, b, dim);
+    else return sqd_ref32(a, b, dim);
+}
+
+// Reference top-2 update rule (decontamination/algo/som_abstract.py:832-853) on (distance, index)
+// candidates visited in any order: the result equals the in-order scan when ties are broken by the
+// lower index. Distances that are NaN or >= 1e99 never enter (the reference starts from 1e99).
+struct Top2 {
+    double d1, d2;
+    int i1, i2;
+    __device__ __forceinline__ void init() { d1 = d2 = 1.0e99; i1 = i2 = -1; }
+    __device__ __forceinline__ static bool less(double da, int ia, double db, int ib)
+    {
+        return da < db || (da == db && (unsigned)ia < (unsigned)ib);
+    }
+    __device__ __forceinline__ void push(double d, int idx)
+    {
+        if (!(d < 1.0e99)) return;  // NaN or too large
+        if (idx == i1 || idx == i2) return;
+        if (less(d, idx, d1, i1)) {
+            d2 = d1; i2 = i1; d1 = d; i1 = idx;
+        } else if (less(d, idx, d2, i2)) {
+            d2 = d; i2 = idx;
+        }
+    }
+};
+
+// ------------------------------------------------------------------------------------------------
+// Host-side structures.
+// ------------------------------------------------------------------------------------------------
+void somhip_set_error(const std::string &msg);
+
+#define SOM_HIP_TRY(expr)                                                                       \
+    do {                                                                                        \
+        hipError_t _e = (expr);                                                                 \
+        if (_e != hipSuccess) {                                                                 \
+            somhip_set_error(std::string(#expr) + ": " + hipGetErrorString(_e));                \
+            return SOMHIP_ERR_HIP;                                                              \
+        }                                                                                       \
+    } while (0)
+
+// Grow-only device buffer.
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    int ensure(size_t need)
+    {
+        if (need <= bytes) return SOMHIP_OK;
+        if (p) SOM_HIP_TRY(hipFree(p));
+        p = nullptr;
+        bytes = 0;
+        SOM_HIP_TRY(hipMalloc(&p, need));
+        bytes = need;
+        return SOMHIP_OK;
+    }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+};
+
+// Per-call device workspace (BMU outputs, reduction partials, fixed-point scales and tables).
+struct Workspace {
+    DevBuf c1, c2, d1, part, bounds, shifts, table;
+    void release()
+    {
+        c1.release(); c2.release(); d1.release(); part.release(); bounds.release(); shifts.release(); table.release();
+    }
+};
+
+// Optional per-kernel timing with HIP events recorded on the context's stream.
+struct KernelTimer {
+    bool enabled = false;
+    struct Pending {
+        int id;
+        hipEvent_t a, b;
+    };
+    std::vector<Pending> pending;
+    std::vector<hipEvent_t> pool;
+    double total_ms[16] = {0};
+    long long launches[16] = {0};
+};
+
+enum TimerId {
+    TK_BMU_EXACT = 0, TK_BMU_FP32, TK_BMU_BF16, TK_BMU_FALLBACK, TK_PREP, TK_ERRORS, TK_ABSMAX, TK_SCATTER,
+    TK_UPDATE, TK_ITER, TK_COUNT
+};
+
+struct somhip_ctx {
+    int device = 0;
+    KernelTimer timer;
+    hipStream_t stream = nullptr;      // stream in use (own or external)
+    hipStream_t own_stream = nullptr;  // stream created by the context
+    // growable device scratch for the BMU fallback list
+    void *scratch = nullptr;
+    size_t scratch_bytes = 0;
+    int num_cus = 256;
+    size_t lds_per_block = 65536;  // device limit of LDS bytes per workgroup
+    Workspace ws;  // inference entry points
+};
+
+// Derived per-weights data prepared by the prep kernels before any BMU search.
+struct somhip_model {
+    somhip_ctx *ctx = nullptr;
+    int m = 0, n = 0, mn = 0, dim = 0, dpad = 0, topology = SOMHIP_TOPOLOGY_HEXAGONAL;
+    int nblk = 0;          // number of 32-neuron blocks (mn rounded up to 32)
+    float *w = nullptr;    // (mn, dim) FP32 weights, row-major
+    float *topo = nullptr; // (mn, 2) FP32 topography
+    // prepared data (valid when prepared == true)
+    bool prepared = false;
+    float *centre = nullptr;        // (dpad) FP32 centroid of the weights
+    float *wmax = nullptr;          // (dpad) max_j |w'_jk|
+    float *scal = nullptr;          // [0] = max_j |w'_j|^2 (as float)
+    float *norms_perm = nullptr;    // (nblk*32) |w'|^2 in permuted slot order (padding slots = +inf)
+    float *norms_nat = nullptr;     // (nblk*32) |w'|^2 in natural order (padding slots = +inf)
+    unsigned short *frag_bf = nullptr;  // bf16 A fragments [blk][kc][hi,lo][64 lanes][8]
+    float *frag_f32 = nullptr;      // FP32 A fragments [blk][dpad/8][64 lanes][4]
+    int *perm = nullptr;            // (nblk*32) slot -> neuron (or -1 for padding)
+};
+
+// Timing helpers: bracket a launch with timer_begin / timer_end (no-ops unless enabled).
+struct TimerScope {
+    somhip_ctx *C;
+    int id;
+    hipEvent_t a = nullptr, b = nullptr;
+    TimerScope(somhip_ctx *c, int i);
+    ~TimerScope();
+};
+int somhip_timer_collect(somhip_ctx *C);
+
+// Internal launchers (return SOMHIP_OK or an error code).
+int somhip_prepare(somhip_model *mdl);
+int somhip_launch_bmu(somhip_model *mdl, const float *x, int64_t n, int precision, int arithmetic, int need_c2,
+                      int default_index, int32_t *c1, int32_t *c2, double *d1);

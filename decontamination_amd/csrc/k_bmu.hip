@@ -1,0 +1,603 @@
+// Best-matching-unit search kernels for gfx950.
+//
+// Three precisions, one output contract: for every sample i the reference's top-2 BMUs
+// (decontamination/algo/som_abstract.py:815-878 update rule = lexicographic (distance, index)),
+// written as c1[i], c2[i] (int32, -1 when absent; c1 falls back to `default_index`) and d1[i]
+// (FP64 squared distance of c1, NaN when absent).
+//
+//   EXACT       bmu_exact_kernel: one sample per lane, weights streamed through LDS, the exact
+//               distance formula (reference FP32-difference/FP64-sum, or FP64) for every neuron.
+//               Also the fallback for samples the MFMA path flags.
+//   FP32        bmu_fp32_kernel: |w'|^2 - 2 x'.w' on v_mfma_f32_32x32x2_f32 (x', w' centred on the
+//               weight centroid), top-2 on those values (the plain FP32 form).
+//   BF16_RERANK bmu_bf16_kernel: the same GEMM on v_mfma_f32_32x32x16_bf16 with a 3-term split
+//               (x_h w_h + x_l w_h + x_h w_l), pass 1 = top-2 of per-lane block minima, pass 2 =
+//               every neuron inside a provable error window of the second minimum, then the exact
+//               distance formula on that candidate set. Samples whose window overflows the
+//               per-lane list, or whose bound is not finite, go to the EXACT fallback.
+#include "common.hpp"
+
+#include <algorithm>
+#include <vector>
+
+namespace {
+
+constexpr int WAVE = 64;
+constexpr int BMU_WAVES = 4;                 // waves per workgroup of the MFMA kernels
+constexpr int CAPL = 8;                      // candidate slots per lane (16 per sample)
+constexpr float U32 = 5.9604644775390625e-8f;  // 2^-24
+
+// Each lane needs its partner (lane ^ 32) value: both hold the same sample column.
+__device__ __forceinline__ float xchg32(float v) { return __shfl_xor(v, 32, WAVE); }
+__device__ __forceinline__ int xchg32i(int v) { return __shfl_xor(v, 32, WAVE); }
+__device__ __forceinline__ double xchg32d(double v) { return __shfl_xor(v, 32, WAVE); }
+
+// ------------------------------------------------------------------------------------------------
+// EXACT: direct-difference scan. Grid: one thread per listed sample.
+// ------------------------------------------------------------------------------------------------
+template <int DP, int MODE, bool NEED_C2>
+__global__ __launch_bounds__(256) void bmu_exact_kernel(const float *__restrict__ x, int64_t n, int dim,
+                                                        const int64_t *__restrict__ list,
+                                                        const unsigned int *__restrict__ list_count,
+                                                        const float *__restrict__ w, int mn, int default_index,
+                                                        int32_t *__restrict__ c1, int32_t *__restrict__ c2,
+                                                        double *__restrict__ d1)
+{
+    constexpr int CH = 128;  // neurons per LDS chunk
+    __shared__ float ws[CH * DP];
+    const int64_t total = list ? (int64_t)(*list_count) : n;
+    // Grid-stride over samples; every thread of the block takes part in the chunk loads.
+    for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < total; base += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t t = base + threadIdx.x;
+        const bool active = t < total;
+        const int64_t i = active ? (list ? list[t] : t) : 0;
+        float xr[DP];
+#pragma unroll
+        for (int k = 0; k < DP; k++) xr[k] = (active && k < dim) ? x[i * dim + k] : 0.f;
+        double md1 = 1.0e99, md2 = 1.0e99;
+        int mi1 = -1, mi2 = -1;
+        for (int j0 = 0; j0 < mn; j0 += CH) {
+            __syncthreads();
+            for (int e = threadIdx.x; e < CH * DP; e += blockDim.x) {
+                int jj = j0 + e / DP, k = e % DP;
+                ws[e] = (jj < mn && k < dim) ? w[(int64_t)jj * dim + k] : 0.f;
+            }
+            __syncthreads();
+            const int jn = min(CH, mn - j0);
+            for (int jj = 0; jj < jn; jj++) {
+                double d = sqd_mode<MODE>(&ws[jj * DP], xr, DP);  // zero padding adds exact zeros
+                if (d != d) continue;
+                const int j = j0 + jj;
+                if (NEED_C2) {
+                    if (md1 > d) {
+                        md2 = md1; mi2 = mi1; md1 = d; mi1 = j;
+                    } else if (md2 > d) {
+                        md2 = d; mi2 = j;
+                    }
+                } else if (md1 > d) {
+                    md1 = d; mi1 = j;
+                }
+            }
+        }
+        if (active) {
+            c1[i] = mi1 >= 0 ? mi1 : default_index;
+            if (c2) c2[i] = mi2;
+            if (d1) d1[i] = mi1 >= 0 ? md1 : __builtin_nan("");
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------------------------
+// Per-sample preparation shared by the MFMA kernels: load this lane's 8 dims (per 16-dim chunk) of
+// the sample, centre them, and compute the quantities of the error bound.
+// ------------------------------------------------------------------------------------------------
+template <int KC>
+struct SampleLoad {
+    float xc[KC][8];  // centred FP32 values of dims kc*16 + 8*(lane/32) + e
+};
+
+template <int KC>
+__device__ __forceinline__ void load_sample(const float *__restrict__ x, int64_t n, int dim, int64_t s, int lane,
+                                            const float *__restrict__ centre, SampleLoad<KC> &L)
+{
+    const int hb = 8 * (lane >> 5);
+#pragma unroll
+    for (int kc = 0; kc < KC; kc++) {
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+            const int k = kc * 16 + hb + e;
+            float v = (s < n && k < dim) ? x[s * dim + k] : 0.f;
+            L.xc[kc][e] = (k < dim) ? __fsub_rn(v, centre[k]) : 0.f;
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------------------------
+// FP32 plain form. Wave = G groups of 32 samples; block = 32 neurons (natural order).
+// ------------------------------------------------------------------------------------------------
+template <int DP, int G>
+__global__ __launch_bounds__(BMU_WAVES * WAVE) void bmu_fp32_kernel(
+    const float *__restrict__ x, int64_t n, int dim, const float *__restrict__ centre,
+    const f32x4 *__restrict__ frag, const f32x4 *__restrict__ norms4, int nblk, int mn, int default_index,
+    int32_t *__restrict__ c1, int32_t *__restrict__ c2, double *__restrict__ d1)
+{
+    constexpr int KS = DP / 2;  // MFMA k-steps of 2
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int wave = threadIdx.x / WAVE;
+    const int64_t sbase = ((int64_t)blockIdx.x * BMU_WAVES + wave) * (G * 32);
+    const int col = lane & 31;
+    const int khalf = lane >> 5;
+
+    // B operand: b[g][s] = -2 * x'[sample][2s + khalf]
+    float b[G][KS];
+    float xn[G];  // |x'|^2 partial (this lane's dims), combined below
+#pragma unroll
+    for (int g = 0; g < G; g++) {
+        const int64_t s = sbase + g * 32 + col;
+        float acc = 0.f;
+#pragma unroll
+        for (int st = 0; st < KS; st++) {
+            const int k = 2 * st + khalf;
+            float v = (s < n && k < dim) ? __fsub_rn(x[s * dim + k], centre[k]) : 0.f;
+            b[g][st] = -2.f * v;
+            acc = fmaf(v, v, acc);
+        }
+        xn[g] = acc + xchg32(acc);
+    }
+
+    float m1[G], m2[G];
+    int i1[G], i2[G];
+#pragma unroll
+    for (int g = 0; g < G; g++) { m1[g] = m2[g] = __builtin_inff(); i1[g] = i2[g] = -1; }
+
+    for (int blk = 0; blk < nblk; blk++) {
+        f32x4 a4[KS / 4];
+#pragma unroll
+        for (int q = 0; q < KS / 4; q++) a4[q] = frag[((int64_t)blk * (KS / 4) + q) * WAVE + lane];
+        f32x16 cin;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            f32x4 t = norms4[blk * 8 + 2 * q + khalf];
+            cin[4 * q + 0] = t[0]; cin[4 * q + 1] = t[1]; cin[4 * q + 2] = t[2]; cin[4 * q + 3] = t[3];
+        }
+        const int jbase = blk * 32 + 4 * khalf;
+#pragma unroll
+        for (int g = 0; g < G; g++) {
+            f32x16 acc = cin;
+#pragma unroll
+            for (int st = 0; st < KS; st++)
+                acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a4[st / 4][st % 4], b[g][st], acc, 0, 0, 0);
+#pragma unroll
+            for (int r = 0; r < 16; r++) {
+                const float v = acc[r];
+                const int j = jbase + 8 * (r >> 2) + (r & 3);
+                const bool lt1 = v < m1[g];
+                const bool lt2 = v < m2[g];
+                m2[g] = lt1 ? m1[g] : (lt2 ? v : m2[g]);
+                i2[g] = lt1 ? i1[g] : (lt2 ? j : i2[g]);
+                m1[g] = lt1 ? v : m1[g];
+                i1[g] = lt1 ? j : i1[g];
+            }
+        }
+    }
+
+#pragma unroll
+    for (int g = 0; g < G; g++) {
+        // merge with the partner lane (lexicographic on (value, index))
+        float pm1 = xchg32(m1[g]), pm2 = xchg32(m2[g]);
+        int pi1 = xchg32i(i1[g]), pi2 = xchg32i(i2[g]);
+        float a1 = m1[g], a2 = m2[g];
+        int j1 = i1[g], j2 = i2[g];
+        auto lessv = [](float da, int ia, float db, int ib) {
+            return da < db || (da == db && (unsigned)ia < (unsigned)ib);
+        };
+        // insert partner's two candidates
+        float cd[2] = {pm1, pm2};
+        int ci[2] = {pi1, pi2};
+#pragma unroll
+        for (int t = 0; t < 2; t++) {
+            if (ci[t] < 0) continue;
+            if (lessv(cd[t], ci[t], a1, j1)) { a2 = a1; j2 = j1; a1 = cd[t]; j1 = ci[t]; }
+            else if (lessv(cd[t], ci[t], a2, j2)) { a2 = cd[t]; j2 = ci[t]; }
+        }
+        const int64_t s = sbase + g * 32 + col;
+        if (lane < 32 && s < n) {
+            const bool ok1 = j1 >= 0 && j1 < mn;
+            c1[s] = ok1 ? j1 : default_index;
+            if (c2) c2[s] = (j2 >= 0 && j2 < mn) ? j2 : -1;
+            if (d1) d1[s] = ok1 ? (double)fmaxf(a1 + xn[g], 0.f) : __builtin_nan("");
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------------------------
+// BF16 3-term split with candidate window and exact rerank.
+// ------------------------------------------------------------------------------------------------
+template <int DP, int G, int MODE, bool NEED_C2>
+__global__ __launch_bounds__(BMU_WAVES * WAVE) void bmu_bf16_kernel(
+    const float *__restrict__ x, int64_t n, int dim, const float *__restrict__ centre,
+    const float *__restrict__ wmax, const float *__restrict__ scal, const bf16x8 *__restrict__ frag,
+    const f32x4 *__restrict__ norms4, const int *__restrict__ perm, int nblk, const float *__restrict__ w, int mn,
+    int default_index, int32_t *__restrict__ c1, int32_t *__restrict__ c2, double *__restrict__ d1,
+    int64_t *__restrict__ fb_list, unsigned int *__restrict__ fb_count)
+{
+    constexpr int KC = DP / 16;
+    __shared__ unsigned short cand[BMU_WAVES][G][CAPL][WAVE];
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int wave = threadIdx.x / WAVE;
+    const int64_t sbase = ((int64_t)blockIdx.x * BMU_WAVES + wave) * (G * 32);
+    const int col = lane & 31;
+    const int khalf = lane >> 5;
+
+    const float nwmax = scal[0];
+    bf16x8 bh[G][KC], bl[G][KC];
+    float E[G], K[G];
+#pragma unroll
+    for (int g = 0; g < G; g++) {
+        const int64_t s = sbase + g * 32 + col;
+        SampleLoad<KC> L;
+        load_sample<KC>(x, n, dim, s, lane, centre, L);
+        float sx = 0.f, kk = 0.f;
+#pragma unroll
+        for (int kc = 0; kc < KC; kc++) {
+#pragma unroll
+            for (int e = 0; e < 8; e++) {
+                const float v = L.xc[kc][e];
+                const unsigned short h = f32_to_bf16_bits(v);
+                const float vh = bf16_bits_to_f32(h);
+                const unsigned short l = f32_to_bf16_bits(v - vh);
+                // -2 scaling is exact in bf16 (exponent shift)
+                bh[g][kc][e] = __builtin_bit_cast(__bf16, f32_to_bf16_bits(-2.f * vh));
+                bl[g][kc][e] = __builtin_bit_cast(__bf16, f32_to_bf16_bits(-2.f * bf16_bits_to_f32(l)));
+                sx = fmaf(fabsf(v), wmax[kc * 16 + 8 * khalf + e], sx);
+                kk = fmaf(v, v, kk);
+            }
+        }
+        sx += xchg32(sx);
+        kk += xchg32(kk);
+        // Error bound of a_j = |w'_j|^2 - 2 x'.w'_j against its exact value (see the design notes):
+        // split truncation 6.04*2^-16*Sx, MFMA FP32 accumulation (3*DP+6)*2^-24*(nwmax + 2.05*Sx),
+        // centring 4*2^-24*(K + nwmax); x1.25 safety; rounding of sx itself covered by the factor.
+        const float split = 6.04f * 1.52587890625e-5f * sx;
+        const float accum = (3.f * DP + 6.f) * U32 * (nwmax + 2.05f * sx);
+        const float cent = 4.f * U32 * (kk + nwmax);
+        E[g] = 1.25f * (split + accum + cent) + 1e-30f;
+        K[g] = kk * (1.f + (DP + 2) * U32);
+    }
+
+    // ---------------- pass 1: top-2 of per-lane block minima ----------------
+    float m1[G], m2[G];
+#pragma unroll
+    for (int g = 0; g < G; g++) m1[g] = m2[g] = __builtin_inff();
+
+    auto block_acc = [&](int blk, int g, const bf16x8 (&ah)[KC], const bf16x8 (&al)[KC], const f32x16 &cin) {
+        f32x16 acc = cin;
+#pragma unroll
+        for (int kc = 0; kc < KC; kc++) {
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[kc], bh[g][kc], acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[kc], bl[g][kc], acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[kc], bh[g][kc], acc, 0, 0, 0);
+        }
+        return acc;
+    };
+    auto load_block = [&](int blk, bf16x8 (&ah)[KC], bf16x8 (&al)[KC], f32x16 &cin) {
+#pragma unroll
+        for (int kc = 0; kc < KC; kc++) {
+            ah[kc] = frag[(((int64_t)blk * KC + kc) * 2 + 0) * WAVE + lane];
+            al[kc] = frag[(((int64_t)blk * KC + kc) * 2 + 1) * WAVE + lane];
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            f32x4 t = norms4[blk * 8 + 2 * q + khalf];
+            cin[4 * q + 0] = t[0]; cin[4 * q + 1] = t[1]; cin[4 * q + 2] = t[2]; cin[4 * q + 3] = t[3];
+        }
+    };
+
+    for (int blk = 0; blk < nblk; blk++) {
+        bf16x8 ah[KC], al[KC];
+        f32x16 cin;
+        load_block(blk, ah, al, cin);
+#pragma unroll
+        for (int g = 0; g < G; g++) {
+            f32x16 acc = block_acc(blk, g, ah, al, cin);
+            float t = __builtin_inff();
+#pragma unroll
+            for (int r = 0; r < 16; r += 2) t = fminf(fminf(t, acc[r]), acc[r + 1]);
+            m2[g] = __builtin_amdgcn_fmed3f(m1[g], m2[g], t);
+            m1[g] = fminf(m1[g], t);
+        }
+    }
+
+    // Threshold: every neuron whose exact distance can be <= the second-best exact distance.
+    float T[G];
+#pragma unroll
+    for (int g = 0; g < G; g++) {
+        const float p1 = xchg32(m1[g]), p2 = xchg32(m2[g]);
+        const float A1 = fminf(m1[g], p1);
+        const float A2 = fminf(fmaxf(m1[g], p1), fminf(m2[g], p2));
+        const float A = NEED_C2 ? A2 : A1;
+        const float scale = fmaxf(A + K[g] + E[g], 0.f);
+        T[g] = A + 2.f * E[g] + 6.2f * U32 * scale + 8.f * U32 * fabsf(A);
+    }
+
+    // ---------------- pass 2: collect the window ----------------
+    int cnt[G];
+#pragma unroll
+    for (int g = 0; g < G; g++) cnt[g] = 0;
+    for (int blk = 0; blk < nblk; blk++) {
+        bf16x8 ah[KC], al[KC];
+        f32x16 cin;
+        load_block(blk, ah, al, cin);
+        const int sbase_blk = blk * 32 + 4 * khalf;
+#pragma unroll
+        for (int g = 0; g < G; g++) {
+            f32x16 acc = block_acc(blk, g, ah, al, cin);
+#pragma unroll
+            for (int r = 0; r < 16; r++) {
+                if (acc[r] <= T[g]) {
+                    if (cnt[g] < CAPL) cand[wave][g][cnt[g]][lane] = (unsigned short)(sbase_blk + 8 * (r >> 2) + (r & 3));
+                    cnt[g]++;
+                }
+            }
+        }
+    }
+
+    // ---------------- exact rerank of the candidate set ----------------
+#pragma unroll
+    for (int g = 0; g < G; g++) {
+        const int64_t s = sbase + g * 32 + col;
+        const bool valid_s = s < n;
+        Top2 tp;
+        tp.init();
+        const int mycnt = cnt[g];
+        const bool ovf = mycnt > CAPL;
+        if (valid_s && !ovf) {
+            float xr[DP];
+#pragma unroll
+            for (int k = 0; k < DP; k++) xr[k] = k < dim ? x[s * dim + k] : 0.f;
+            for (int e = 0; e < mycnt; e++) {
+                const int slot = cand[wave][g][e][lane];
+                const int j = perm[slot];
+                if (j < 0) continue;
+                float wr[DP];
+#pragma unroll
+                for (int k = 0; k < DP; k++) wr[k] = k < dim ? w[(int64_t)j * dim + k] : 0.f;
+                tp.push(sqd_mode<MODE>(wr, xr, DP), j);
+            }
+        }
+        // merge with partner lane
+        const double pd1 = xchg32d(tp.d1), pd2 = xchg32d(tp.d2);
+        const int pi1 = xchg32i(tp.i1), pi2 = xchg32i(tp.i2);
+        if (pi1 >= 0) tp.push(pd1, pi1);
+        if (pi2 >= 0) tp.push(pd2, pi2);
+        const bool povf = xchg32i(ovf ? 1 : 0) != 0;
+        const bool bad = ovf || povf || !(T[g] < __builtin_inff()) || (T[g] != T[g]);
+        if (lane < 32 && valid_s) {
+            if (bad) {
+                const unsigned int slot = atomicAdd(fb_count, 1u);
+                fb_list[slot] = s;
+            } else {
+                c1[s] = tp.i1 >= 0 ? tp.i1 : default_index;
+                if (c2) c2[s] = tp.i2;
+                if (d1) d1[s] = tp.i1 >= 0 ? tp.d1 : __builtin_nan("");
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------------------------
+// Preparation of the derived weight data.
+// ------------------------------------------------------------------------------------------------
+__global__ void prep_centre_kernel(const float *__restrict__ w, int mn, int dim, int dpad, float *__restrict__ centre)
+{
+    // one block per padded dimension; fixed-order FP64 sum
+    const int k = blockIdx.x;
+    __shared__ double red[256];
+    double acc = 0.0;
+    if (k < dim)
+        for (int j = threadIdx.x; j < mn; j += blockDim.x) acc += (double)w[(int64_t)j * dim + k];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) centre[k] = k < dim ? (float)(red[0] / (double)mn) : 0.f;
+}
+
+__global__ void prep_rows_kernel(const float *__restrict__ w, int mn, int dim, int dpad, int nblk,
+                                 const float *__restrict__ centre, const int *__restrict__ perm,
+                                 unsigned short *__restrict__ frag_bf, float *__restrict__ frag_f32,
+                                 float *__restrict__ norms_perm, float *__restrict__ norms_nat,
+                                 unsigned int *__restrict__ wmax_bits, unsigned int *__restrict__ nwmax_bits)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;  // slot (permuted) == neuron slot (natural)
+    if (p >= nblk * 32) return;
+    const int kcn = dpad / 16;
+    const int ks4 = dpad / 8;
+    // permuted slot -> bf16 fragments and permuted norms
+    {
+        const int j = perm[p];
+        float nrm = 0.f;
+        const int blk = p >> 5, row = p & 31;
+        for (int d = 0; d < dpad; d++) {
+            float v = (j >= 0 && d < dim) ? __fsub_rn(w[(int64_t)j * dim + d], centre[d]) : 0.f;
+            nrm = __fmaf_rn(v, v, nrm);
+            const unsigned short h = f32_to_bf16_bits(v);
+            const unsigned short l = f32_to_bf16_bits(__fsub_rn(v, bf16_bits_to_f32(h)));
+            const int kc = d >> 4, half = (d & 15) >> 3, e = d & 7;
+            const int lane = row + 32 * half;
+            frag_bf[((((int64_t)blk * kcn + kc) * 2 + 0) * 64 + lane) * 8 + e] = h;
+            frag_bf[((((int64_t)blk * kcn + kc) * 2 + 1) * 64 + lane) * 8 + e] = l;
+            if (j >= 0) atomicMax(&wmax_bits[d], __builtin_bit_cast(unsigned int, fabsf(v)));
+        }
+        norms_perm[p] = j >= 0 ? nrm : __builtin_inff();
+        if (j >= 0) atomicMax(nwmax_bits, __builtin_bit_cast(unsigned int, nrm));
+    }
+    // natural slot -> FP32 fragments and natural norms
+    {
+        const int j = p < mn ? p : -1;
+        float nrm = 0.f;
+        const int blk = p >> 5, row = p & 31;
+        for (int d = 0; d < dpad; d++) {
+            float v = (j >= 0 && d < dim) ? __fsub_rn(w[(int64_t)j * dim + d], centre[d]) : 0.f;
+            nrm = __fmaf_rn(v, v, nrm);
+            const int st = d >> 1, half = d & 1;
+            const int lane = row + 32 * half;
+            frag_f32[(((int64_t)blk * ks4 + (st >> 2)) * 64 + lane) * 4 + (st & 3)] = v;
+        }
+        norms_nat[p] = j >= 0 ? nrm : __builtin_inff();
+    }
+}
+
+// Launch helpers -----------------------------------------------------------------------------------
+
+template <int DP>
+void launch_exact(int mode, bool need_c2, hipStream_t st, const float *x, int64_t n, int dim, const int64_t *list,
+                  const unsigned int *list_count, int64_t grid_items, const float *w, int mn, int default_index,
+                  int32_t *c1, int32_t *c2, double *d1)
+{
+    int64_t blocks = (grid_items + 255) / 256;
+    if (blocks < 1) blocks = 1;
+    if (blocks > 65535 * 8) blocks = 65535 * 8;
+    dim3 grid((unsigned)blocks), block(256);
+#define SOM_EX(M, C)                                                                                            \
+    hipLaunchKernelGGL((bmu_exact_kernel<DP, M, C>), grid, block, 0, st, x, n, dim, list, list_count, w, mn,  \
+                       default_index, c1, c2, d1)
+    if (mode == SOMHIP_RERANK_FP64) {
+        if (need_c2) SOM_EX(SOMHIP_RERANK_FP64, true); else SOM_EX(SOMHIP_RERANK_FP64, false);
+    } else {
+        if (need_c2) SOM_EX(SOMHIP_RERANK_REF32, true); else SOM_EX(SOMHIP_RERANK_REF32, false);
+    }
+#undef SOM_EX
+}
+
+constexpr int G_FP32 = 2;
+constexpr int G_BF16 = 2;
+
+template <int DP>
+void launch_fp32(hipStream_t st, somhip_model *M, const float *x, int64_t n, int default_index, int32_t *c1,
+                 int32_t *c2, double *d1)
+{
+    const int64_t per_block = (int64_t)BMU_WAVES * G_FP32 * 32;
+    const int64_t blocks = (n + per_block - 1) / per_block;
+    hipLaunchKernelGGL((bmu_fp32_kernel<DP, G_FP32>), dim3((unsigned)blocks), dim3(BMU_WAVES * WAVE), 0, st, x, n,
+                       M->dim, M->centre, (const f32x4 *)M->frag_f32, (const f32x4 *)M->norms_nat, M->nblk, M->mn,
+                       default_index, c1, c2, d1);
+}
+
+template <int DP>
+void launch_bf16(int mode, bool need_c2, hipStream_t st, somhip_model *M, const float *x, int64_t n,
+                 int default_index, int32_t *c1, int32_t *c2, double *d1, int64_t *fb_list, unsigned int *fb_count)
+{
+    const int64_t per_block = (int64_t)BMU_WAVES * G_BF16 * 32;
+    const int64_t blocks = (n + per_block - 1) / per_block;
+    dim3 grid((unsigned)blocks), block(BMU_WAVES * WAVE);
+#define SOM_BF(MO, C)                                                                                           \
+    hipLaunchKernelGGL((bmu_bf16_kernel<DP, G_BF16, MO, C>), grid, block, 0, st, x, n, M->dim, M->centre, M->wmax, \
+                       M->scal, (const bf16x8 *)M->frag_bf, (const f32x4 *)M->norms_perm, M->perm, M->nblk, M->w,   \
+                       M->mn, default_index, c1, c2, d1, fb_list, fb_count)
+    if (mode == SOMHIP_RERANK_FP64) {
+        if (need_c2) SOM_BF(SOMHIP_RERANK_FP64, true); else SOM_BF(SOMHIP_RERANK_FP64, false);
+    } else {
+        if (need_c2) SOM_BF(SOMHIP_RERANK_REF32, true); else SOM_BF(SOMHIP_RERANK_REF32, false);
+    }
+#undef SOM_BF
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------------------------------------
+// Internal entry points.
+// ------------------------------------------------------------------------------------------------
+int somhip_prepare(somhip_model *M)
+{
+    if (M->prepared) return SOMHIP_OK;
+    hipStream_t st = M->ctx->stream;
+    TimerScope ts(M->ctx, TK_PREP);
+    hipLaunchKernelGGL(prep_centre_kernel, dim3(M->dpad), dim3(256), 0, st, M->w, M->mn, M->dim, M->dpad, M->centre);
+    SOM_HIP_TRY(hipMemsetAsync(M->wmax, 0, sizeof(float) * M->dpad, st));
+    SOM_HIP_TRY(hipMemsetAsync(M->scal, 0, sizeof(float) * 4, st));
+    const int slots = M->nblk * 32;
+    hipLaunchKernelGGL(prep_rows_kernel, dim3((slots + 255) / 256), dim3(256), 0, st, M->w, M->mn, M->dim, M->dpad,
+                       M->nblk, M->centre, M->perm, M->frag_bf, M->frag_f32, M->norms_perm, M->norms_nat,
+                       (unsigned int *)M->wmax, (unsigned int *)M->scal);
+    SOM_HIP_TRY(hipGetLastError());
+    M->prepared = true;
+    return SOMHIP_OK;
+}
+
+// Workspace for the BF16 fallback list (grow-only, per context).
+static int fb_workspace(somhip_ctx *C, int64_t n, int64_t **list, unsigned int **count)
+{
+    size_t need = sizeof(int64_t) * (size_t)(n > 0 ? n : 1) + 256;
+    if (C->scratch_bytes < need) {
+        if (C->scratch) SOM_HIP_TRY(hipFree(C->scratch));
+        C->scratch = nullptr;
+        C->scratch_bytes = 0;
+        SOM_HIP_TRY(hipMalloc(&C->scratch, need));
+        C->scratch_bytes = need;
+    }
+    *count = (unsigned int *)C->scratch;
+    *list = (int64_t *)((char *)C->scratch + 256);
+    return SOMHIP_OK;
+}
+
+int somhip_launch_bmu(somhip_model *M, const float *x, int64_t n, int precision, int arithmetic, int need_c2,
+                      int default_index, int32_t *c1, int32_t *c2, double *d1)
+{
+    if (n <= 0) return SOMHIP_OK;
+    hipStream_t st = M->ctx->stream;
+    const int dp = M->dpad;
+    if (precision == SOMHIP_PREC_EXACT) {
+        TimerScope ts(M->ctx, TK_BMU_EXACT);
+        switch (dp) {
+            case 16: launch_exact<16>(arithmetic, need_c2, st, x, n, M->dim, nullptr, nullptr, n, M->w, M->mn, default_index, c1, c2, d1); break;
+            case 32: launch_exact<32>(arithmetic, need_c2, st, x, n, M->dim, nullptr, nullptr, n, M->w, M->mn, default_index, c1, c2, d1); break;
+            case 48: launch_exact<48>(arithmetic, need_c2, st, x, n, M->dim, nullptr, nullptr, n, M->w, M->mn, default_index, c1, c2, d1); break;
+            case 64: launch_exact<64>(arithmetic, need_c2, st, x, n, M->dim, nullptr, nullptr, n, M->w, M->mn, default_index, c1, c2, d1); break;
+            default: somhip_set_error("dim > 64 is not supported"); return SOMHIP_ERR_UNSUPPORTED;
+        }
+        SOM_HIP_TRY(hipGetLastError());
+        return SOMHIP_OK;
+    }
+    if (dp > 32) {
+        somhip_set_error("MFMA precisions support dim <= 32; use precision EXACT");
+        return SOMHIP_ERR_UNSUPPORTED;
+    }
+    int rc = somhip_prepare(M);
+    if (rc) return rc;
+    if (precision == SOMHIP_PREC_FP32) {
+        TimerScope ts(M->ctx, TK_BMU_FP32);
+        if (dp == 16) launch_fp32<16>(st, M, x, n, default_index, c1, c2, d1);
+        else launch_fp32<32>(st, M, x, n, default_index, c1, c2, d1);
+        SOM_HIP_TRY(hipGetLastError());
+        return SOMHIP_OK;
+    }
+    if (precision != SOMHIP_PREC_BF16_RERANK) {
+        somhip_set_error("unknown precision");
+        return SOMHIP_ERR_ARG;
+    }
+    if (M->nblk * 32 > 65536) {
+        somhip_set_error("BF16_RERANK supports at most 65536 neurons");
+        return SOMHIP_ERR_UNSUPPORTED;
+    }
+    int64_t *fb_list;
+    unsigned int *fb_count;
+    rc = fb_workspace(M->ctx, n, &fb_list, &fb_count);
+    if (rc) return rc;
+    SOM_HIP_TRY(hipMemsetAsync(fb_count, 0, sizeof(unsigned int), st));
+    {
+        TimerScope ts(M->ctx, TK_BMU_BF16);
+        if (dp == 16) launch_bf16<16>(arithmetic, need_c2, st, M, x, n, default_index, c1, c2, d1, fb_list, fb_count);
+        else launch_bf16<32>(arithmetic, need_c2, st, M, x, n, default_index, c1, c2, d1, fb_list, fb_count);
+        SOM_HIP_TRY(hipGetLastError());
+    }
+    TimerScope tf(M->ctx, TK_BMU_FALLBACK);
+    // Fallback over the flagged samples: a fixed grid that reads the device-side count.
+    const int64_t fb_items = std::min<int64_t>(n, (int64_t)M->ctx->num_cus * 256);
+    if (dp == 16) launch_exact<16>(arithmetic, need_c2, st, x, n, M->dim, fb_list, fb_count, fb_items, M->w, M->mn, default_index, c1, c2, d1);
+    else launch_exact<32>(arithmetic, need_c2, st, x, n, M->dim, fb_list, fb_count, fb_items, M->w, M->mn, default_index, c1, c2, d1);
+    SOM_HIP_TRY(hipGetLastError());
+    return SOMHIP_OK;
+}

@@ -1,0 +1,462 @@
+// Training, activation and error-reduction kernels for gfx950.
+//
+// Per-BMU sums use exact fixed-point integers: every contribution rho_i * x_ik is scaled by 2^shift_k
+// (shift_k chosen per chunk so that the chunk's total cannot overflow int64) and rounded once; the
+// sums are then exact integer sums, which do not depend on the order of accumulation. Each workgroup
+// accumulates a contiguous sample range into LDS with 64-bit LDS atomics (a slice of neurons at a
+// time), writes its slice table, and a reduction kernel adds the workgroup tables (int64) and scales
+// the result into the FP64 accumulator. Results are therefore bit-identical for any launch shape,
+// batching or chunking into the same calls.
+#include "common.hpp"
+
+#include <algorithm>
+
+namespace {
+
+constexpr size_t LDS_SCATTER_MAX = 160 * 1024;
+
+// ------------------------------------------------------------------------------------------------
+// Column-wise max |v| over finite values; out[k] for k < dim is max|x_k|, out[dim] is max rho (as
+// float after the optional float rounding). Stored as float bits with atomicMax (values >= 0).
+// ------------------------------------------------------------------------------------------------
+template <int DP>
+__global__ __launch_bounds__(256) void absmax_kernel(const float *__restrict__ x, int dim, const double *__restrict__ rho,
+                                                    int rho_f32, int64_t n, unsigned int *__restrict__ out)
+{
+    float mx[DP];
+#pragma unroll
+    for (int k = 0; k < DP; k++) mx[k] = 0.f;
+    double mr = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        if (x) {
+#pragma unroll
+            for (int k = 0; k < DP; k++) {
+                if (k < dim) {
+                    float v = fabsf(x[i * dim + k]);
+                    if (v < __builtin_inff()) mx[k] = fmaxf(mx[k], v);
+                }
+            }
+        }
+        if (rho) {
+            double r = rho[i];
+            if (rho_f32) r = (double)(float)r;
+            if (r > 0.0 && r < 1e300) mr = fmax(mr, r);
+        }
+    }
+    __shared__ unsigned int red[DP + 1];
+    if (threadIdx.x <= DP) red[threadIdx.x] = 0u;
+    __syncthreads();
+    if (x) {
+#pragma unroll
+        for (int k = 0; k < DP; k++)
+            if (k < dim) atomicMax(&red[k], __builtin_bit_cast(unsigned int, mx[k]));
+    }
+    // rho bound rounded up to a float so the scale stays safe
+    float mrf = (float)mr;
+    if ((double)mrf < mr) mrf = nextafterf(mrf, __builtin_inff());
+    atomicMax(&red[DP], __builtin_bit_cast(unsigned int, mrf));
+    __syncthreads();
+    if ((int)threadIdx.x < dim && x) atomicMax(&out[threadIdx.x], red[threadIdx.x]);
+    if (threadIdx.x == 0) atomicMax(&out[dim], red[DP]);
+}
+
+// shift_k = floor(61 - log2(B_k * n)): every |rho x| * 2^shift <= 2^61 / n, so the int64 sum of n
+// terms (each rounded once) stays below 2^62. Column dim uses the rho bound (1 when rho is absent).
+__global__ void shifts_kernel(const unsigned int *__restrict__ bounds, int dim, int has_x, int has_rho, int64_t n,
+                              int *__restrict__ shifts)
+{
+    const int k = threadIdx.x;
+    if (k > dim) return;
+    const double rb = has_rho ? (double)__builtin_bit_cast(float, bounds[dim]) : 1.0;
+    double B = (k == dim) ? rb : (has_x ? (double)__builtin_bit_cast(float, bounds[k]) * rb : 0.0);
+    int s = 0;
+    if (B > 0.0) {
+        const double t = B * (double)(n > 0 ? n : 1);
+        int e;
+        frexp(t, &e);  // t < 2^e
+        s = 61 - e;
+        if (s > 1000) s = 1000;
+        if (s < -1000) s = -1000;
+    }
+    shifts[k] = s;
+}
+
+// ------------------------------------------------------------------------------------------------
+// Fixed-point scatter of per-sample rows into per-neuron integer sums.
+//   nv = dim + 1 columns (x != NULL: rho*x_0..rho*x_{dim-1}, rho) or nv = 1 (x == NULL: rho).
+//   Samples with bmu < 0 or !(rho > 0) are skipped. rho_f32: round rho to float first (training).
+// ------------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(512) void scatter_fixed_kernel(const int32_t *__restrict__ bmu,
+                                                            const float *__restrict__ x, int dim,
+                                                            const double *__restrict__ rho, int rho_f32, int64_t n,
+                                                            int64_t per_wg, const int *__restrict__ shifts, int mn,
+                                                            int nv, int slice_len,
+                                                            unsigned long long *__restrict__ partial)
+{
+    extern __shared__ unsigned long long lds[];
+    const int64_t i0 = (int64_t)blockIdx.x * per_wg;
+    const int64_t i1 = min(n, i0 + per_wg);
+    unsigned long long *out = partial + (int64_t)blockIdx.x * mn * nv;
+    for (int c0 = 0; c0 < mn; c0 += slice_len) {
+        const int cl = min(slice_len, mn - c0);
+        for (int e = threadIdx.x; e < cl * nv; e += blockDim.x) lds[e] = 0ull;
+        __syncthreads();
+        for (int64_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+            const int c = bmu[i];
+            if (c < c0 || c >= c0 + cl) continue;
+            double r = rho ? rho[i] : 1.0;
+            if (rho_f32) r = (double)(float)r;
+            if (!(r > 0.0) || !(r < 1e300)) continue;
+            unsigned long long *row = lds + (int64_t)(c - c0) * nv;
+            if (x) {
+                for (int k = 0; k < dim; k++) {
+                    const double v = r * (double)x[i * dim + k];
+                    const long long q = llrint(ldexp(v, shifts[k]));
+                    atomicAdd(&row[k], (unsigned long long)q);
+                }
+            }
+            const long long qr = llrint(ldexp(r, shifts[nv - 1]));
+            atomicAdd(&row[nv - 1], (unsigned long long)qr);
+        }
+        __syncthreads();
+        for (int e = threadIdx.x; e < cl * nv; e += blockDim.x) out[(int64_t)c0 * nv + e] = lds[e];
+        __syncthreads();
+    }
+}
+
+// acc[e] += 2^-shift * sum_b partial[b][e] (integer sum, exact)
+__global__ void scatter_reduce_kernel(const unsigned long long *__restrict__ partial, int nwg, int64_t width, int nv,
+                                      const int *__restrict__ shifts, double *__restrict__ acc)
+{
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= width) return;
+    unsigned long long s = 0ull;
+    for (int b = 0; b < nwg; b++) s += partial[(int64_t)b * width + e];
+    const int k = (int)(e % nv);
+    acc[e] += ldexp((double)(long long)s, -shifts[k]);
+}
+
+// ------------------------------------------------------------------------------------------------
+// Epoch-mode update: part[p][j][k] = sum_{c in split p} H_jc S_c[k], H_jc = exp(-|t_j-t_c|^2 inv2s2).
+// ------------------------------------------------------------------------------------------------
+template <int NV>
+__global__ __launch_bounds__(256) void update_partial_kernel(const double *__restrict__ S, const float *__restrict__ topo,
+                                                             int mn, int nv, double inv_2s2, int chunk,
+                                                             double *__restrict__ part)
+{
+    constexpr int TC = 64;
+    __shared__ double s_S[TC * NV];
+    __shared__ float s_t[TC * 2];
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int p = blockIdx.y;
+    const int cbeg = p * chunk, cend = min(mn, cbeg + chunk);
+    float tj[2] = {0.f, 0.f};
+    if (j < mn) { tj[0] = topo[2 * j]; tj[1] = topo[2 * j + 1]; }
+    double acc[NV];
+#pragma unroll
+    for (int k = 0; k < NV; k++) acc[k] = 0.0;
+    for (int c0 = cbeg; c0 < cend; c0 += TC) {
+        const int cl = min(TC, cend - c0);
+        __syncthreads();
+        for (int e = threadIdx.x; e < cl * nv; e += blockDim.x) s_S[(e / nv) * NV + e % nv] = S[(int64_t)c0 * nv + e];
+        for (int e = threadIdx.x; e < cl * 2; e += blockDim.x) s_t[e] = topo[2 * c0 + e];
+        __syncthreads();
+        for (int cc = 0; cc < cl; cc++) {
+            const double cnt = s_S[cc * NV + nv - 1];
+            if (cnt == 0.0) continue;  // no sample has this BMU: S_c = 0, nothing to add
+            const double h = exp(-sqd_ref32(tj, &s_t[2 * cc], 2) * inv_2s2);
+#pragma unroll
+            for (int k = 0; k < NV; k++)
+                if (k < nv) acc[k] = __fma_rn(h, s_S[cc * NV + k], acc[k]);
+        }
+    }
+    if (j < mn) {
+#pragma unroll
+        for (int k = 0; k < NV; k++)
+            if (k < nv) part[((int64_t)p * mn + j) * nv + k] = acc[k];
+    }
+}
+
+// w_j = num_j / den_j if den_j > 1e-12 else 0 (som_batch.py:354-359); num/den = fixed-order sums of
+// the split partials (or the accumulator itself when nsplit == 1 and part == acc).
+__global__ void divide_kernel(const double *__restrict__ part, int nsplit, int mn, int dim, float *__restrict__ w,
+                              float *__restrict__ hist)
+{
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (int64_t)mn * dim) return;
+    const int j = (int)(e / dim), k = (int)(e % dim);
+    const int nv = dim + 1;
+    double num = 0.0, den = 0.0;
+    for (int p = 0; p < nsplit; p++) {
+        num += part[((int64_t)p * mn + j) * nv + k];
+        den += part[((int64_t)p * mn + j) * nv + dim];
+    }
+    const float v = den > SOM_FLOAT_EPS ? (float)(num / den) : 0.f;
+    w[e] = v;
+    if (hist) hist[e] = v;
+}
+
+// ------------------------------------------------------------------------------------------------
+// Vector-count mode: part[p][j][k] = sum_{i in split p} rho_i exp(-|t_j - t_bmu_i|^2 / (2 sigma_i^2)) x_ik
+// with sigma_i = sigma0 / (1 + 2 (cur + i) / n_vectors) >= 1e-12 (som_batch.py:154-210).
+// ------------------------------------------------------------------------------------------------
+template <int DP>
+__global__ __launch_bounds__(256) void iter_partial_kernel(const int32_t *__restrict__ bmu, const float *__restrict__ x,
+                                                           int dim, const double *__restrict__ rho, int64_t n,
+                                                           int64_t cur, int64_t nvec, float sigma0,
+                                                           const float *__restrict__ topo, int mn, int64_t chunk,
+                                                           double *__restrict__ part)
+{
+    constexpr int TS = 64;
+    __shared__ float s_x[TS * DP];
+    __shared__ float s_tb[TS * 2];
+    __shared__ double s_h0[TS];   // rho_i (float-rounded, as the reference casts density to dtype)
+    __shared__ double s_inv[TS];  // 1 / (2 sigma_i^2)
+    __shared__ int s_ok[TS];
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int p = blockIdx.y;
+    const int64_t ibeg = (int64_t)p * chunk, iend = min(n, ibeg + chunk);
+    float tj[2] = {0.f, 0.f};
+    if (j < mn) { tj[0] = topo[2 * j]; tj[1] = topo[2 * j + 1]; }
+    double acc[DP + 1];
+#pragma unroll
+    for (int k = 0; k <= DP; k++) acc[k] = 0.0;
+    for (int64_t i0 = ibeg; i0 < iend; i0 += TS) {
+        const int il = (int)min((int64_t)TS, iend - i0);
+        __syncthreads();
+        for (int e = threadIdx.x; e < il * DP; e += blockDim.x) {
+            const int ii = e / DP, k = e % DP;
+            s_x[e] = k < dim ? x[(i0 + ii) * dim + k] : 0.f;
+        }
+        for (int ii = threadIdx.x; ii < il; ii += blockDim.x) {
+            const int64_t i = i0 + ii;
+            const int c = bmu[i];
+            const float r = rho ? (float)rho[i] : 1.0f;
+            const bool ok = c >= 0 && r > 0.0f;
+            s_ok[ii] = ok;
+            s_tb[2 * ii] = ok ? topo[2 * c] : 0.f;
+            s_tb[2 * ii + 1] = ok ? topo[2 * c + 1] : 0.f;
+            double sg = (double)sigma0 * (1.0 / (1.0 + 2.0 * (double)(cur + i) / (double)nvec));
+            if (sg < SOM_FLOAT_EPS) sg = SOM_FLOAT_EPS;
+            s_inv[ii] = 1.0 / (2.0 * sg * sg);
+            s_h0[ii] = (double)r;
+        }
+        __syncthreads();
+        for (int ii = 0; ii < il; ii++) {
+            if (!s_ok[ii]) continue;
+            const double h = s_h0[ii] * exp(-sqd_ref32(tj, &s_tb[2 * ii], 2) * s_inv[ii]);
+#pragma unroll
+            for (int k = 0; k < DP; k++)
+                if (k < dim) acc[k] = __fma_rn(h, (double)s_x[ii * DP + k], acc[k]);
+            acc[DP] += h;
+        }
+    }
+    if (j < mn) {
+        const int nv = dim + 1;
+#pragma unroll
+        for (int k = 0; k < DP; k++)
+            if (k < dim) part[((int64_t)p * mn + j) * nv + k] = acc[k];
+        part[((int64_t)p * mn + j) * nv + dim] = acc[DP];
+    }
+}
+
+// acc[e] += sum_p part[p][e] (fixed order)
+__global__ void sum_parts_kernel(const double *__restrict__ part, int nsplit, int64_t width, double *__restrict__ acc)
+{
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= width) return;
+    double s = 0.0;
+    for (int p = 0; p < nsplit; p++) s += part[(int64_t)p * width + e];
+    acc[e] += s;
+}
+
+// ------------------------------------------------------------------------------------------------
+// Errors (som_abstract.py:815-878): per-workgroup fixed-order sums, then one fixed-order final sum.
+// ------------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void errors_partial_kernel(const int32_t *__restrict__ c1, const int32_t *__restrict__ c2,
+                                                             const double *__restrict__ d1, const double *__restrict__ rho,
+                                                             int64_t n, const float *__restrict__ topo, double penalty,
+                                                             int64_t per_wg, double *__restrict__ part)
+{
+    __shared__ double red[3][256];
+    const int64_t i0 = (int64_t)blockIdx.x * per_wg, i1 = min(n, i0 + per_wg);
+    double q = 0.0, t = 0.0, s = 0.0;
+    for (int64_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+        const double r = rho ? rho[i] : 1.0;
+        if (!(r > 0.0)) continue;
+        const int a = c1[i], b = c2[i];
+        if (a < 0 || b < 0 || a == b) continue;
+        q += r * sqrt(d1[i]);
+        if (sqd_ref32(&topo[2 * a], &topo[2 * b], 2) > penalty) t += r;
+        s += r;
+    }
+    red[0][threadIdx.x] = q; red[1][threadIdx.x] = t; red[2][threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o)
+            for (int k = 0; k < 3; k++) red[k][threadIdx.x] += red[k][threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x < 3) part[(int64_t)blockIdx.x * 3 + threadIdx.x] = red[threadIdx.x][0];
+}
+
+__global__ __launch_bounds__(256) void errors_final_kernel(const double *__restrict__ part, int nparts,
+                                                           double *__restrict__ errors)
+{
+    __shared__ double red[3][256];
+    double v[3] = {0.0, 0.0, 0.0};
+    for (int p = threadIdx.x; p < nparts; p += blockDim.x)
+        for (int k = 0; k < 3; k++) v[k] += part[(int64_t)p * 3 + k];
+    for (int k = 0; k < 3; k++) red[k][threadIdx.x] = v[k];
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o)
+            for (int k = 0; k < 3; k++) red[k][threadIdx.x] += red[k][threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x < 3) errors[threadIdx.x] += red[threadIdx.x][0];
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------------------------------------
+// Host launchers used by capi.hip.
+// ------------------------------------------------------------------------------------------------
+int somhip_launch_absmax(hipStream_t st, const float *x, int dim, int dpad, const double *rho, int rho_f32, int64_t n,
+                         unsigned int *bounds, int num_cus)
+{
+    SOM_HIP_TRY(hipMemsetAsync(bounds, 0, sizeof(unsigned int) * (dim + 1), st));
+    if (n <= 0) return SOMHIP_OK;
+    const int64_t blocks = std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, (int64_t)num_cus * 8));
+    dim3 g((unsigned)blocks), b(256);
+    switch (dpad) {
+        case 16: hipLaunchKernelGGL(absmax_kernel<16>, g, b, 0, st, x, dim, rho, rho_f32, n, bounds); break;
+        case 32: hipLaunchKernelGGL(absmax_kernel<32>, g, b, 0, st, x, dim, rho, rho_f32, n, bounds); break;
+        case 48: hipLaunchKernelGGL(absmax_kernel<48>, g, b, 0, st, x, dim, rho, rho_f32, n, bounds); break;
+        default: hipLaunchKernelGGL(absmax_kernel<64>, g, b, 0, st, x, dim, rho, rho_f32, n, bounds); break;
+    }
+    SOM_HIP_TRY(hipGetLastError());
+    return SOMHIP_OK;
+}
+
+int somhip_launch_shifts(hipStream_t st, const unsigned int *bounds, int dim, int has_x, int has_rho, int64_t n,
+                         int *shifts)
+{
+    hipLaunchKernelGGL(shifts_kernel, dim3(1), dim3(128), 0, st, bounds, dim, has_x, has_rho, n, shifts);
+    SOM_HIP_TRY(hipGetLastError());
+    return SOMHIP_OK;
+}
+
+// Number of scatter workgroups so that the partial table stays within `table_budget` bytes.
+int somhip_scatter_nwg(int64_t n, int mn, int nv, int num_cus, size_t table_budget)
+{
+    int64_t by_budget = (int64_t)(table_budget / ((size_t)mn * nv * 8));
+    int64_t by_n = (n + 2047) / 2048;
+    int64_t w = std::min<int64_t>(std::min<int64_t>(by_budget, by_n), (int64_t)num_cus);
+    return (int)std::max<int64_t>(1, w);
+}
+
+int somhip_launch_scatter(hipStream_t st, const int32_t *bmu, const float *x, int dim, const double *rho, int rho_f32,
+                          int64_t n, const int *shifts, int mn, int nv, int nwg, unsigned long long *partial,
+                          double *acc, size_t lds_limit)
+{
+    if (n <= 0) return SOMHIP_OK;
+    const size_t lds_bytes = std::min(LDS_SCATTER_MAX, lds_limit);
+    int slice_len = (int)(lds_bytes / ((size_t)nv * 8));
+    if (slice_len < 1) {
+        somhip_set_error("LDS too small for one neuron row");
+        return SOMHIP_ERR_UNSUPPORTED;
+    }
+    if (slice_len > mn) slice_len = mn;
+    const size_t lds = (size_t)slice_len * nv * 8;
+    SOM_HIP_TRY(hipFuncSetAttribute((const void *)scatter_fixed_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)lds));
+    const int64_t per_wg = (n + nwg - 1) / nwg;
+    hipLaunchKernelGGL(scatter_fixed_kernel, dim3(nwg), dim3(512), lds, st, bmu, x, dim, rho, rho_f32, n, per_wg,
+                       shifts, mn, nv, slice_len, partial);
+    SOM_HIP_TRY(hipGetLastError());
+    const int64_t width = (int64_t)mn * nv;
+    hipLaunchKernelGGL(scatter_reduce_kernel, dim3((unsigned)((width + 255) / 256)), dim3(256), 0, st, partial, nwg,
+                       width, nv, shifts, acc);
+    SOM_HIP_TRY(hipGetLastError());
+    return SOMHIP_OK;
+}
+
+int somhip_update_nsplit(int mn)
+{
+    // enough (neuron, split) threads to fill the device: about 256K
+    int s = (int)std::max<int64_t>(1, (int64_t)262144 / std::max(mn, 1));
+    return std::min(s, std::max(1, mn / 16));
+}
+
+int somhip_launch_update(hipStream_t st, const double *S, const float *topo, int mn, int dim, double sigma,
+                         double *part, int nsplit, float *w, float *hist)
+{
+    const int nv = dim + 1;
+    const double inv_2s2 = 1.0 / (2.0 * sigma * sigma);
+    const int chunk = (mn + nsplit - 1) / nsplit;
+    dim3 g((mn + 255) / 256, nsplit), b(256);
+    if (nv <= 17) hipLaunchKernelGGL(update_partial_kernel<17>, g, b, 0, st, S, topo, mn, nv, inv_2s2, chunk, part);
+    else if (nv <= 33) hipLaunchKernelGGL(update_partial_kernel<33>, g, b, 0, st, S, topo, mn, nv, inv_2s2, chunk, part);
+    else if (nv <= 49) hipLaunchKernelGGL(update_partial_kernel<49>, g, b, 0, st, S, topo, mn, nv, inv_2s2, chunk, part);
+    else hipLaunchKernelGGL(update_partial_kernel<65>, g, b, 0, st, S, topo, mn, nv, inv_2s2, chunk, part);
+    SOM_HIP_TRY(hipGetLastError());
+    const int64_t tot = (int64_t)mn * dim;
+    hipLaunchKernelGGL(divide_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, part, nsplit, mn, dim, w,
+                       hist);
+    SOM_HIP_TRY(hipGetLastError());
+    return SOMHIP_OK;
+}
+
+int somhip_launch_divide(hipStream_t st, const double *acc, int mn, int dim, float *w, float *hist)
+{
+    const int64_t tot = (int64_t)mn * dim;
+    hipLaunchKernelGGL(divide_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, acc, 1, mn, dim, w, hist);
+    SOM_HIP_TRY(hipGetLastError());
+    return SOMHIP_OK;
+}
+
+int somhip_iter_nsplit(int64_t n, int mn)
+{
+    int64_t s = std::max<int64_t>(1, (int64_t)262144 / std::max(mn, 1));
+    s = std::min<int64_t>(s, std::max<int64_t>(1, n / 64));
+    return (int)std::min<int64_t>(s, 4096);
+}
+
+int somhip_launch_iter(hipStream_t st, const int32_t *bmu, const float *x, int dim, int dpad, const double *rho,
+                       int64_t n, int64_t cur, int64_t nvec, float sigma0, const float *topo, int mn, double *part,
+                       int nsplit, double *acc)
+{
+    if (n <= 0) return SOMHIP_OK;
+    const int64_t chunk = (n + nsplit - 1) / nsplit;
+    dim3 g((mn + 255) / 256, nsplit), b(256);
+#define SOM_IT(D) hipLaunchKernelGGL(iter_partial_kernel<D>, g, b, 0, st, bmu, x, dim, rho, n, cur, nvec, sigma0, topo, mn, chunk, part)
+    switch (dpad) {
+        case 16: SOM_IT(16); break;
+        case 32: SOM_IT(32); break;
+        case 48: SOM_IT(48); break;
+        default: SOM_IT(64); break;
+    }
+#undef SOM_IT
+    SOM_HIP_TRY(hipGetLastError());
+    const int64_t width = (int64_t)mn * (dim + 1);
+    hipLaunchKernelGGL(sum_parts_kernel, dim3((unsigned)((width + 255) / 256)), dim3(256), 0, st, part, nsplit, width,
+                       acc);
+    SOM_HIP_TRY(hipGetLastError());
+    return SOMHIP_OK;
+}
+
+int somhip_errors_nparts(int64_t n) { return (int)std::max<int64_t>(1, std::min<int64_t>(4096, (n + 4095) / 4096)); }
+
+int somhip_launch_errors(hipStream_t st, const int32_t *c1, const int32_t *c2, const double *d1, const double *rho,
+                         int64_t n, const float *topo, double penalty, double *part, int nparts, double *errors)
+{
+    if (n <= 0) return SOMHIP_OK;
+    const int64_t per_wg = (n + nparts - 1) / nparts;
+    hipLaunchKernelGGL(errors_partial_kernel, dim3(nparts), dim3(256), 0, st, c1, c2, d1, rho, n, topo, penalty, per_wg,
+                       part);
+    SOM_HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(errors_final_kernel, dim3(1), dim3(256), 0, st, part, nparts, errors);
+    SOM_HIP_TRY(hipGetLastError());
+    return SOMHIP_OK;
+}

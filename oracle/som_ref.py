@@ -1,0 +1,224 @@
+"""Restated host driver of the reference SOM path on top of the C oracle kernels.
+
+TEST INFRASTRUCTURE ONLY (see ``oracle/__init__.py``).
+
+Follows, step by step:
+  * topography        decontamination/algo/som_abstract.py:81-111
+  * init_rand         decontamination/algo/som_abstract.py:115-132
+  * compute_errors    decontamination/algo/som_abstract.py:528-601
+  * activation map    decontamination/algo/som_abstract.py:605-666
+  * get_winners       decontamination/algo/som_abstract.py:670-695
+  * SOM_Batch.train   decontamination/algo/som_batch.py:214-505 (epoch and vector-count modes,
+                      history, per-epoch errors, early stop, best epoch)
+  * chunk protocol    decontamination/algo/__init__.py:21-27
+
+``factorised_epoch_fp64`` / ``top2_fp64_np`` are a fast FP64 oracle for inputs too large for the
+per-sample C loop; they are pinned against the C FP64 kernels in tests/test_oracle.py.
+"""
+import numpy as np
+
+from . import FP64, REF32, f32, f64, lib, ptr
+
+FLOAT_EPS = 1.0e-12
+
+
+def topography(m, n, topology="hexagonal", dtype=np.float32):
+    """som_abstract.py:81-111: index i*n+j; square [i, j]; hexagonal [i + 0.5*(j&1), j*sqrt(3)/2]."""
+    if topology == "square":
+        pts = [[i, j] for i in range(m) for j in range(n)]
+    else:
+        pts = [[i + (j & 1) * 0.5, j * 0.8660254037844386] for i in range(m) for j in range(n)]
+    return np.array(pts, dtype=dtype).reshape(m * n, 2)
+
+
+def init_rand(m, n, dim, seed=None, dtype=np.float32):
+    """som_abstract.py:128-132."""
+    return np.random.default_rng(seed=seed).random((m * n, dim), dtype=dtype)
+
+
+def _chunks(dataset):
+    """algo/__init__.py:21-27: ndarray -> one chunk; callable builder -> builder()()."""
+    if dataset is None:
+        return None
+    if callable(dataset):
+        return dataset()()
+    return iter((dataset,))
+
+
+def find_bmus(W, X, mode=REF32):
+    W = f32(W); X = f32(X)
+    out = np.empty(X.shape[0], np.int32)
+    lib().orc_find_bmus(mode, ptr(W), W.shape[0], W.shape[1], ptr(X), X.shape[0], ptr(out))
+    return out
+
+
+def activation_map(W, X, rho=None, mode=REF32, m=None, n=None):
+    W = f32(W); X = f32(X); rho = f64(rho)
+    res = np.zeros(W.shape[0], np.float64)
+    lib().orc_count_bmus(mode, ptr(W), W.shape[0], W.shape[1], ptr(X), ptr(rho), X.shape[0], ptr(res))
+    return res if m is None else res.reshape(m, n)
+
+
+def errors_raw(W, topo, X, rho=None, topology="hexagonal", mode=REF32):
+    W = f32(W); X = f32(X); rho = f64(rho); topo = f32(topo)
+    res = np.zeros(3, np.float64)
+    lib().orc_compute_errors(mode, ptr(W), ptr(topo), W.shape[0], W.shape[1], ptr(X), ptr(rho), X.shape[0],
+                             2.0 if topology == "square" else 1.0, ptr(res))
+    return res
+
+
+def compute_errors(W, topo, dataset, dataset_weights=None, topology="hexagonal", mode=REF32):
+    """som_abstract.py:547-601 (NaN pair when mn < 2 or the weight sum is 0)."""
+    if W.shape[0] < 2:
+        return np.nan, np.nan
+    res = np.zeros(3, np.float64)
+    dch = _chunks(dataset)
+    wch = _chunks(dataset_weights)
+    pairs = zip(dch, wch) if wch is not None else ((c, None) for c in dch)
+    for xc, rc in pairs:
+        res += errors_raw(W, topo, xc, rc, topology, mode)
+    return (res[0] / res[2], res[1] / res[2]) if res[2] > 0.0 else (np.nan, np.nan)
+
+
+def top2(W, topo, X, rho=None, mode=REF32):
+    W = f32(W); X = f32(X); rho = f64(rho); topo = f32(topo)
+    c1 = np.empty(X.shape[0], np.int32); c2 = np.empty(X.shape[0], np.int32)
+    lib().orc_top2(mode, ptr(W), ptr(topo), W.shape[0], W.shape[1], ptr(X), ptr(rho), X.shape[0], ptr(c1), ptr(c2))
+    return c1, c2
+
+
+def epoch_accumulate(W, topo, X, rho, cur, total, sigma0, mode=REF32, iter_mode=False, acc=None):
+    """One launch of _train_step1_{epoch,iter}_kernel over a chunk; returns (num, den)."""
+    W = f32(W); X = f32(X); rho = f64(rho); topo = f32(topo)
+    mn, dim = W.shape
+    dt = np.float64 if mode == FP64 else np.float32
+    if acc is None:
+        acc = (np.zeros((mn, dim), dt), np.zeros(mn, dt))
+    num, den = acc
+    fn = lib().orc_train_iter_accumulate if iter_mode else lib().orc_train_epoch_accumulate
+    fn(mode, ptr(num), ptr(den), ptr(W), ptr(topo), ptr(X), ptr(rho), X.shape[0], cur, total,
+       np.float32(sigma0), mn, dim)
+    return acc
+
+
+def divide(num, den, mode=REF32):
+    mn, dim = num.shape
+    out = np.empty((mn, dim), np.float32)
+    lib().orc_divide(mode, ptr(num), ptr(den), mn, dim, ptr(out))
+    return out
+
+
+def train(W0, m, n, dataset, dataset_weights=None, n_epochs=None, n_vectors=None, sigma=None,
+          topology="hexagonal", use_best_epoch=True, stop_quantization_error=None, stop_topographic_error=None,
+          mode=REF32):
+    """Restatement of SOM_Batch.train (som_batch.py:214-505). Returns (weights, history, qe, te)."""
+    sigma = max(m, n) / 2.0 if sigma is None else float(sigma)
+    sigma0 = np.float32(sigma)
+    topo = topography(m, n, topology)
+    W = f32(W0).copy()
+    mn, dim = W.shape
+    sq = -1.0e6 if stop_quantization_error is None else stop_quantization_error
+    st = -1.0e6 if stop_topographic_error is None else stop_topographic_error
+
+    def chunk_pairs():
+        dch = _chunks(dataset)
+        wch = _chunks(dataset_weights)
+        if wch is None:
+            for c in dch:
+                yield c, None
+        else:
+            for c, r in zip(dch, wch):
+                if c.shape[0] != r.shape[0]:
+                    raise ValueError("`dataset` and `dataset_weights` chunks must be aligned")
+                yield c, r
+
+    if n_epochs is not None and n_vectors is None:
+        history = np.empty((n_epochs, mn, dim), np.float32)
+        qe = np.full(n_epochs, np.nan); te = np.full(n_epochs, np.nan)
+        for e in range(n_epochs):
+            acc = None
+            for c, r in chunk_pairs():
+                acc = epoch_accumulate(W, topo, c, r, e, n_epochs, sigma0, mode, False, acc)
+            W = history[e] = divide(*acc, mode)
+            qe[e], te[e] = compute_errors(W, topo, dataset, dataset_weights, topology, mode)
+            if qe[e] <= sq and te[e] <= st:
+                break
+    elif n_epochs is None and n_vectors is not None:
+        history = np.empty((1, mn, dim), np.float32)
+        qe = np.zeros(1); te = np.zeros(1)
+        acc = None
+        cur = 0
+        for c, r in chunk_pairs():
+            count = min(c.shape[0], n_vectors - cur)
+            acc = epoch_accumulate(W, topo, c[:count], None if r is None else r[:count], cur, n_vectors, sigma0,
+                                   mode, True, acc)
+            cur += count
+            if cur >= n_vectors:
+                break
+        W = history[0] = divide(*acc, mode)
+        qe[0], te[0] = compute_errors(W, topo, dataset, dataset_weights, topology, mode)
+    else:
+        raise ValueError("Invalid training method, specify either `n_epochs` or `n_vectors`")
+    if use_best_epoch:
+        W = history[np.nanargmin(qe)].copy()
+    return W, history, qe, te
+
+
+# ---------------------------------------------------------------------------------------------
+# Fast factorised FP64 oracle (large inputs).
+# ---------------------------------------------------------------------------------------------
+
+def top2_fp64_np(W, X, block=8192, k=4):
+    """FP64 top-2 BMUs with lowest-index tie-break: expanded-form FP64 screen, then the direct FP64
+    formula on the k best candidates. Returns (c1, c2, d1) with d1 the direct FP64 squared distance.
+    Pinned against orc_top2(FP64) in tests."""
+    W64 = np.asarray(W, np.float32).astype(np.float64)
+    nw = (W64 * W64).sum(1)
+    N = X.shape[0]
+    c1 = np.empty(N, np.int64); c2 = np.empty(N, np.int64); d1 = np.empty(N)
+    for s in range(0, N, block):
+        x = np.asarray(X[s:s + block], np.float32).astype(np.float64)
+        d = nw[None, :] - 2.0 * (x @ W64.T)
+        cand = np.argpartition(d, k, axis=1)[:, :k]
+        diff = x[:, None, :] - W64[cand]
+        dd = (diff * diff).sum(-1)
+        # lexicographic (distance, index) per row: stable sort by index, then by distance
+        key_idx = np.argsort(cand, axis=1, kind="stable")
+        cand_s = np.take_along_axis(cand, key_idx, 1)
+        dd_s = np.take_along_axis(dd, key_idx, 1)
+        o = np.argsort(dd_s, axis=1, kind="stable")
+        c1[s:s + block] = np.take_along_axis(cand_s, o[:, :1], 1)[:, 0]
+        c2[s:s + block] = np.take_along_axis(cand_s, o[:, 1:2], 1)[:, 0]
+        d1[s:s + block] = np.take_along_axis(dd_s, o[:, :1], 1)[:, 0]
+    return c1, c2, d1
+
+
+def neighbourhood_fp64(topo, sigma):
+    """H[j, c] = exp(-|t_j - t_c|^2 / (2 sigma^2)) with the reference's FP32 topography differences."""
+    t = np.asarray(topo, np.float32)
+    diff = (t[:, None, :] - t[None, :, :]).astype(np.float32)
+    sq = (diff * diff).astype(np.float32).astype(np.float64).sum(-1)
+    return np.exp(-sq * (1.0 / (2.0 * sigma * sigma)))
+
+
+def sigma_at(sigma0, e, etot):
+    s = float(np.float32(sigma0)) * (1.0 / (1.0 + 2.0 * e / etot))
+    return max(s, FLOAT_EPS)
+
+
+def factorised_epoch_fp64(W, topo, X, rho, sigma, bmu=None):
+    """One FP64 batch epoch in the factorised form num = H.S, den = H.c (algebraically identical to
+    som_batch.py:509-566 in epoch mode because h depends only on (j, bmu(i))). Returns
+    (W_new float32, bmu, c2, d1)."""
+    mn, dim = np.asarray(W).shape
+    c1, c2, d1 = top2_fp64_np(W, X) if bmu is None else (bmu, None, None)
+    r = np.ones(X.shape[0]) if rho is None else np.asarray(rho, np.float64)
+    keep = r > 0
+    S = np.zeros((mn, dim)); cnt = np.zeros(mn)
+    np.add.at(S, c1[keep], np.asarray(X, np.float32)[keep].astype(np.float64) * r[keep, None])
+    np.add.at(cnt, c1[keep], r[keep])
+    H = neighbourhood_fp64(topo, sigma)
+    num = H @ S
+    den = H @ cnt
+    Wn = np.where(den[:, None] > FLOAT_EPS, num / np.where(den[:, None] > FLOAT_EPS, den[:, None], 1.0), 0.0)
+    return Wn.astype(np.float32), c1, c2, d1

@@ -1,0 +1,260 @@
+"""GPU parity: the HIP path (through the C ABI) against the reference's golden vectors and the CPU
+oracle on the same seeded inputs.
+
+Bars: BMU indices and hit counts bit-exact against the oracle's reference arithmetic ('exact' and
+'bf16_rerank' precisions; 'fp32' is the expanded-form GEMM, whose near-tie choices are reported
+against a tolerance instead); QE to 1e-12 relative; TE and activation counts exact; trained
+centroids within the reference's own rtol 1e-4 of the oracle and of the golden vectors, and within
+1e-6 relative of the FP64 factorised oracle on the same BMUs (one-step check).
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import decontamination_amd as dam
+from oracle import FP64, REF32, build as build_oracle
+from oracle import som_ref as R
+
+pytestmark = pytest.mark.gpu
+
+GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "reference_vectors.json")))
+PRECS = ["exact", "fp32", "bf16_rerank"]
+EXACT_PRECS = ["exact", "bf16_rerank"]
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _setup():
+    build_oracle()
+    assert dam.gpu_available(), "no GPU visible: the GPU tests must run on an MI355X"
+
+
+def golden_data(n):
+    return np.random.default_rng(seed=0).random((n, 4), np.float32)
+
+
+def golden_som(cls=dam.SOM_Batch, **kw):
+    som = cls(4, 4, 4, dtype=np.float32, topology="square", **kw)
+    som.init_rand(seed=0)
+    return som
+
+
+# ---------------------------------------------------------------- golden vectors -------------
+
+@pytest.mark.parametrize("precision", PRECS)
+def test_golden_winners(precision):
+    som = golden_som(precision=precision)
+    assert np.array_equal(som.get_winners(golden_data(8), threads_per_blocks=8), np.array(GOLD["winners"]))
+
+
+@pytest.mark.parametrize("precision", PRECS)
+def test_golden_activation_map(precision):
+    som = golden_som(precision=precision)
+    act = som.get_activation_map(golden_data(25_000), threads_per_blocks=8)
+    assert act.dtype == np.float32 and act.shape == (4, 4)
+    assert np.array_equal(act, np.array(GOLD["activation_map"], np.float32))
+
+
+@pytest.mark.parametrize("precision", PRECS)
+def test_golden_epoch_centroids(precision):
+    som = golden_som(precision=precision)
+    som.train(golden_data(1000), n_epochs=1, threads_per_blocks=64)
+    assert np.allclose(som.centroids, np.array(GOLD["centroids_epoch"]), rtol=GOLD["centroids_epoch_rtol"])
+
+
+@pytest.mark.parametrize("precision", PRECS)
+def test_golden_iter_centroids(precision):
+    som = golden_som(precision=precision)
+    som.train(golden_data(1000), n_vectors=1000, threads_per_blocks=64)
+    assert np.allclose(som.centroids, np.array(GOLD["centroids_iter"]), rtol=GOLD["centroids_iter_rtol"])
+
+
+@pytest.mark.parametrize("precision", PRECS)
+def test_golden_errors_smoke(precision):
+    som = golden_som(precision=precision)
+    q, t = som.compute_errors(golden_data(8), threads_per_blocks=8)
+    assert q == 0.0 or np.isfinite(q)
+
+
+# ---------------------------------------------------------------- BMU parity -----------------
+
+CASES = [
+    # (m, n, dim, N, topology, seed)
+    (1, 1, 4, 50, "square", 1),
+    (1, 2, 3, 200, "hexagonal", 2),
+    (4, 4, 4, 1000, "square", 3),
+    (5, 7, 8, 3001, "hexagonal", 4),
+    (10, 10, 8, 5000, "square", 5),
+    (8, 12, 16, 4099, "hexagonal", 6),
+    (16, 16, 17, 2048, "hexagonal", 7),
+    (64, 64, 16, 6000, "hexagonal", 8),
+    (20, 30, 32, 3000, "square", 9),
+    (7, 9, 40, 700, "square", 10),
+]
+
+
+def make_case(m, n, dim, N, seed, ties=True, nans=True):
+    rng = np.random.default_rng(seed)
+    W = rng.standard_normal((m * n, dim)).astype(np.float32)
+    X = rng.standard_normal((N, dim)).astype(np.float32)
+    if ties and m * n >= 4:
+        W[m * n // 2] = W[1]  # duplicated neuron: equal distances, lower index must win
+        X[: min(N, 5)] = W[1]
+    if nans and N > 10:
+        X[3, 0] = np.nan
+        X[9] = np.nan
+    return W, X
+
+
+@pytest.mark.parametrize("case", CASES, ids=[f"{c[0]}x{c[1]}d{c[2]}n{c[3]}{c[4][0]}" for c in CASES])
+@pytest.mark.parametrize("precision", PRECS)
+def test_winners_vs_oracle(case, precision):
+    m, n, dim, N, topo, seed = case
+    if precision != "exact" and dim > 32:
+        pytest.skip("MFMA precisions support dim <= 32")
+    W, X = make_case(m, n, dim, N, seed)
+    som = dam.SOM_Batch(m, n, dim, topology=topo, precision=precision)
+    som._weights[...] = W
+    got = som.get_winners(X)
+    exp = R.find_bmus(W, X, REF32)
+    assert got.dtype == np.int32 and got.shape == (N,)
+    if precision in EXACT_PRECS:
+        assert np.array_equal(got, exp)
+    else:
+        agree = np.mean(got == exp)
+        assert agree >= 0.99, agree
+        assert np.array_equal(got[:5], exp[:5]) or m * n < 4  # exact ties resolved to the lower index
+
+
+@pytest.mark.parametrize("case", CASES, ids=[f"{c[0]}x{c[1]}d{c[2]}n{c[3]}{c[4][0]}" for c in CASES])
+@pytest.mark.parametrize("precision", EXACT_PRECS)
+def test_errors_vs_oracle(case, precision):
+    m, n, dim, N, topo, seed = case
+    if precision != "exact" and dim > 32:
+        pytest.skip("MFMA precisions support dim <= 32")
+    W, X = make_case(m, n, dim, N, seed)
+    rho = np.random.default_rng(seed + 100).random(N)
+    rho[::13] = 0.0
+    som = dam.SOM_Batch(m, n, dim, topology=topo, precision=precision)
+    som._weights[...] = W
+    q, t = som.compute_errors(X, rho)
+    eq, et = R.compute_errors(W, R.topography(m, n, topo), X, rho, topo, REF32)
+    if m * n < 2:
+        assert np.isnan(q) and np.isnan(t)
+        return
+    assert np.isclose(q, eq, rtol=1e-12, atol=0), (q, eq)
+    assert np.isclose(t, et, rtol=1e-12, atol=0), (t, et)
+
+
+@pytest.mark.parametrize("case", CASES[:8], ids=[f"{c[0]}x{c[1]}d{c[2]}n{c[3]}{c[4][0]}" for c in CASES[:8]])
+@pytest.mark.parametrize("precision", EXACT_PRECS)
+def test_activation_vs_oracle(case, precision):
+    m, n, dim, N, topo, seed = case
+    W, X = make_case(m, n, dim, N, seed)
+    som = dam.SOM_Batch(m, n, dim, topology=topo, precision=precision)
+    som._weights[...] = W
+    got = som.get_activation_map(X)
+    exp = R.activation_map(W, X, None, REF32, m, n)
+    assert np.array_equal(got, exp.astype(np.float32))
+    rho = np.random.default_rng(seed).random(N) * 3.0
+    rho[::5] = -1.0
+    got = som.get_activation_map(X, rho)
+    exp = R.activation_map(W, X, rho, REF32, m, n)
+    assert np.allclose(got, exp.astype(np.float32), rtol=1e-6, atol=0)
+
+
+@pytest.mark.parametrize("precision", EXACT_PRECS)
+def test_fp64_arithmetic_matches_fp64_oracle(precision):
+    W, X = make_case(12, 12, 16, 5000, 21)
+    som = dam.SOM_Batch(12, 12, 16, precision=precision, arithmetic="fp64")
+    som._weights[...] = W
+    assert np.array_equal(som.get_winners(X), R.find_bmus(W, X, FP64))
+
+
+# ---------------------------------------------------------------- training parity ------------
+
+@pytest.mark.parametrize("precision", PRECS)
+@pytest.mark.parametrize("topology", ["square", "hexagonal"])
+def test_train_epochs_vs_oracle(precision, topology):
+    m, n, dim, N = 6, 7, 5, 4000
+    rng = np.random.default_rng(31)
+    X = rng.standard_normal((N, dim)).astype(np.float32)
+    rho = rng.random(N)
+    W0 = rng.standard_normal((m * n, dim)).astype(np.float32)
+    som = dam.SOM_Batch(m, n, dim, topology=topology, precision=precision)
+    som._weights[...] = W0
+    som.train(X, rho, n_epochs=3, use_best_epoch=False)
+    Wr, hist, qe, te = R.train(W0, m, n, X, rho, n_epochs=3, topology=topology, use_best_epoch=False, mode=REF32)
+    assert som._history.shape == (3, m * n, dim)
+    if precision == "fp32":
+        # expanded-form FP32 BMUs may differ from the reference formula at near ties
+        assert np.allclose(som.weights, Wr, rtol=0, atol=2e-2)
+        assert np.allclose(som.quantization_errors, qe, rtol=1e-3)
+        return
+    assert np.allclose(som.weights, Wr, rtol=1e-4, atol=1e-6)
+    assert np.allclose(som.quantization_errors, qe, rtol=1e-5)
+    assert np.allclose(som.topographic_errors, te, rtol=1e-9)
+
+
+@pytest.mark.parametrize("precision", EXACT_PRECS)
+def test_one_step_vs_fp64_factorised(precision):
+    """Same weights in, one epoch: GPU centroids vs the FP64 factorised oracle on the same BMUs."""
+    m, n, dim, N = 16, 16, 16, 20000
+    rng = np.random.default_rng(41)
+    X = rng.standard_normal((N, dim)).astype(np.float32)
+    W0 = rng.standard_normal((m * n, dim)).astype(np.float32) * 0.3
+    som = dam.SOM_Batch(m, n, dim, precision=precision, arithmetic="fp64")
+    som._weights[...] = W0
+    som.train(X, n_epochs=1, use_best_epoch=False)
+    sigma = R.sigma_at(max(m, n) / 2.0, 0, 1)
+    Wf, c1, _, _ = R.factorised_epoch_fp64(W0, R.topography(m, n), X, None, sigma)
+    rel = np.abs(som.weights - Wf) / np.maximum(np.abs(Wf), 1e-3)
+    assert rel.max() <= 1e-6, rel.max()
+
+
+def test_determinism_and_chunking():
+    m, n, dim, N = 10, 10, 8, 30000
+    rng = np.random.default_rng(51)
+    X = rng.standard_normal((N, dim)).astype(np.float32)
+    rho = rng.random(N)
+    W0 = rng.standard_normal((m * n, dim)).astype(np.float32)
+    res = []
+    for _ in range(2):
+        som = dam.SOM_Batch(m, n, dim)
+        som._weights[...] = W0
+        som.train(X, rho, n_epochs=2, use_best_epoch=False)
+        res.append(som.weights.copy())
+    assert np.array_equal(res[0], res[1])
+    gen = lambda a: (lambda: lambda: (a[i:i + 7000] for i in range(0, len(a), 7000)))
+    som = dam.SOM_Batch(m, n, dim)
+    som._weights[...] = W0
+    som.train(gen(X), gen(rho), n_epochs=2, use_best_epoch=False)
+    assert np.allclose(som.weights, res[0], rtol=1e-12, atol=1e-12)
+
+
+def test_early_stop_and_best_epoch():
+    m, n, dim, N = 5, 5, 4, 3000
+    rng = np.random.default_rng(61)
+    X = rng.random((N, dim)).astype(np.float32)
+    W0 = rng.random((m * n, dim)).astype(np.float32)
+    som = dam.SOM_Batch(m, n, dim)
+    som._weights[...] = W0
+    som.train(X, n_epochs=6, stop_quantization_error=1e9, stop_topographic_error=1e9, use_best_epoch=False)
+    assert np.isfinite(som.quantization_errors[0]) and np.all(np.isnan(som.quantization_errors[1:]))
+    Wr, hist, qe, te = R.train(W0, m, n, X, n_epochs=6, stop_quantization_error=1e9,
+                               stop_topographic_error=1e9, use_best_epoch=False)
+    assert np.allclose(som.weights, Wr, rtol=1e-4, atol=1e-6)
+    som2 = dam.SOM_Batch(m, n, dim)
+    som2._weights[...] = W0
+    som2.train(X, n_epochs=4)
+    assert np.array_equal(som2.weights, som2._history[np.nanargmin(som2.quantization_errors)])
+
+
+def test_empty_and_invalid_inputs():
+    som = dam.SOM_Batch(3, 3, 4)
+    som.init_rand(0)
+    assert som.get_winners(np.zeros((0, 4), np.float32)).shape == (0,)
+    assert np.array_equal(som.get_activation_map(np.zeros((0, 4), np.float32)), np.zeros((3, 3), np.float32))
+    with pytest.raises(ValueError):
+        som.get_winners(np.zeros((5, 3), np.float32))

@@ -1,0 +1,238 @@
+"""Pins the CPU oracle (oracle/) before anything is checked against it.
+
+Parity status of each oracle function (what pins it):
+  find_bmus / _find_bmu_xpu      golden winners (test_som_abstract.py:57-60), golden activation counts,
+                                 brute force (numpy argmin with lexicographic ties, NaN rows).
+  count_bmus                     golden activation map (test_som_abstract.py:41-46), sum invariant,
+                                 integer-weight == duplication invariant.
+  compute_errors / top2          brute force top-2 by numpy sort, closed forms (x_i = w_j gives QE = 0;
+                                 constructed TE cases on square and hexagonal maps), rho linearity.
+  train epoch step + divide      golden epoch centroids (test_som_batch.py:30-50, rtol 1e-4), closed
+                                 forms (sigma -> 1e-12 clamp gives the k-means step; sigma huge gives
+                                 the global mean), chunking invariance.
+  train iter step                golden vector-count centroids (test_som_batch.py:63-83, rtol 1e-4).
+  factorised FP64 epoch / top2   agreement with the C FP64 kernels on random inputs.
+  HEALPix pixel-centre z         the reference's own local-coordinate formula restated independently
+                                 (hp/__init__.py:376-419 with u = v = 0.5), equal-area symmetry.
+QE/TE values on hexagonal maps have no reference number (the reference's test asserts nothing,
+test_som_abstract.py:68-73): they are pinned by the closed forms and brute force above only.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from oracle import FP64, REF32, build as build_oracle
+from oracle import som_ref as R
+
+GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "reference_vectors.json")))
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _built():
+    build_oracle()
+
+
+def golden_data(n, seed=0):
+    return np.random.default_rng(seed=seed).random((n, 4), np.float32)
+
+
+@pytest.mark.parametrize("mode", [REF32, FP64])
+def test_golden_epoch_centroids(mode):
+    W0 = R.init_rand(4, 4, 4, seed=0)
+    W, hist, qe, te = R.train(W0, 4, 4, golden_data(1000), n_epochs=1, topology="square", mode=mode)
+    assert np.allclose(W.reshape(4, 4, 4), np.array(GOLD["centroids_epoch"]), rtol=GOLD["centroids_epoch_rtol"])
+
+
+@pytest.mark.parametrize("mode", [REF32, FP64])
+def test_golden_iter_centroids(mode):
+    W0 = R.init_rand(4, 4, 4, seed=0)
+    W, hist, qe, te = R.train(W0, 4, 4, golden_data(1000), n_vectors=1000, topology="square", mode=mode)
+    assert np.allclose(W.reshape(4, 4, 4), np.array(GOLD["centroids_iter"]), rtol=GOLD["centroids_iter_rtol"])
+
+
+def test_golden_activation_and_winners():
+    W0 = R.init_rand(4, 4, 4, seed=0)
+    act = R.activation_map(W0, golden_data(25_000), m=4, n=4)
+    assert np.array_equal(act, np.array(GOLD["activation_map"], dtype=np.float64))
+    assert np.array_equal(R.find_bmus(W0, golden_data(8)), np.array(GOLD["winners"]))
+
+
+def test_init_rand_matches_dataset_prefix():
+    # init_rand(0) draws the same stream as the golden datasets: its rows are data points
+    assert np.array_equal(R.init_rand(4, 4, 4, seed=0), golden_data(16))
+
+
+def _brute_top2(W, X):
+    """Independent top-2: numpy FP64-sum of FP32 squared differences, lexicographic (d, index)."""
+    diff = (X[:, None, :].astype(np.float32) - W[None, :, :].astype(np.float32)).astype(np.float32)
+    d = (diff * diff).astype(np.float32).astype(np.float64)
+    d = np.add.reduce(d, axis=2)  # FP64 accumulation
+    d = np.where(np.isnan(d), np.inf, d)
+    idx = np.broadcast_to(np.arange(W.shape[0]), d.shape)
+    order = np.lexsort((idx, d), axis=1)
+    c1 = order[:, 0]
+    c2 = order[:, 1] if W.shape[0] > 1 else np.full(X.shape[0], -1)
+    d1 = np.take_along_axis(d, order[:, :1], 1)[:, 0]
+    c1 = np.where(np.isfinite(d1), c1, -1)
+    return c1, c2, d1
+
+
+def test_find_bmus_brute_force_with_ties_and_nans():
+    rng = np.random.default_rng(3)
+    W = rng.integers(0, 3, (40, 3)).astype(np.float32)  # many exact duplicates -> ties
+    X = rng.integers(0, 3, (500, 3)).astype(np.float32)
+    X[7] = np.nan
+    c1, c2, _ = _brute_top2(W, X)
+    got = R.find_bmus(W, X)
+    exp = np.where(c1 < 0, 0, c1)
+    assert np.array_equal(got, exp)
+    g1, g2 = R.top2(W, R.topography(5, 8, "square"), X)
+    assert np.array_equal(g1[c1 >= 0], c1[c1 >= 0])
+    assert np.array_equal(g2[c1 >= 0], c2[c1 >= 0])
+    assert g1[7] == -1 and g2[7] == -1
+
+
+def test_errors_closed_forms_square():
+    # samples placed exactly on neurons: QE = 0. Each sample's second BMU is chosen by construction.
+    m, n = 3, 3
+    topo = R.topography(m, n, "square")
+    W = (topo * 10.0).astype(np.float32)  # neuron j at 10 * grid position, far apart
+    x0 = W[4] + np.array([1.0, 0.0], np.float32)  # near centre neuron, second = (1,2) right neighbour
+    x1 = W[0] + np.array([0.0, 0.0], np.float32)  # on neuron 0; second = neighbours at distance 10 (tie -> 1)
+    X = np.stack([x0, x1]).astype(np.float32)
+    e = R.errors_raw(W, topo, X, None, "square")
+    assert e[2] == 2.0
+    assert np.isclose(e[0], 1.0)  # sqrt(1) + sqrt(0)
+    # neighbours (1,1)-(1,2) and (0,0)-(0,1) are at squared distance 1 <= 2: no topographic error
+    assert e[1] == 0.0
+    # move x1 so that its second BMU is the diagonal-far neuron (2,2): |t|^2 = 8 > 2
+    W2 = W.copy()
+    W2[8] = W2[0] + np.array([0.5, 0.5], np.float32)
+    e2 = R.errors_raw(W2, topo, X[1:], None, "square")
+    assert e2[1] == 1.0 and e2[2] == 1.0
+
+
+def test_errors_hex_threshold_uses_fp32_topography():
+    # hexagonal diagonal neighbours: squared topographic distance in the reference arithmetic can be
+    # slightly above 1.0 -> counted as topographic error (strict '>')
+    topo = R.topography(4, 4, "hexagonal")
+    t = topo.astype(np.float32)
+    d = ((t[1] - t[0]).astype(np.float32) ** 2).astype(np.float64).sum()
+    W = np.zeros((16, 2), np.float32)
+    W[0] = [0.0, 0.0]
+    W[1] = [1.0, 0.0]
+    W[2:] = 100.0
+    X = np.array([[0.1, 0.0]], np.float32)
+    e = R.errors_raw(W, topo, X, None, "hexagonal")
+    assert e[1] == (1.0 if d > 1.0 else 0.0)
+
+
+def test_errors_weight_linearity_and_skips():
+    rng = np.random.default_rng(5)
+    W = rng.random((20, 3), np.float32)
+    topo = R.topography(4, 5, "hexagonal")
+    X = rng.random((300, 3), np.float32)
+    rho = rng.random(300)
+    rho[::7] = 0.0
+    rho[::11] = -1.0
+    e1 = R.errors_raw(W, topo, X, rho, "hexagonal")
+    e2 = R.errors_raw(W, topo, X, 2.0 * rho, "hexagonal")
+    assert np.allclose(e2, 2.0 * e1, rtol=1e-12)
+    keep = rho > 0
+    e3 = R.errors_raw(W, topo, X[keep], rho[keep], "hexagonal")
+    assert np.allclose(e1, e3, rtol=1e-12)
+
+
+def test_activation_duplication_invariant():
+    rng = np.random.default_rng(6)
+    W = rng.random((12, 5), np.float32)
+    X = rng.random((400, 5), np.float32)
+    k = rng.integers(0, 4, 400)
+    a = R.activation_map(W, X, k.astype(np.float64))
+    b = R.activation_map(W, np.repeat(X, k, axis=0))
+    assert np.array_equal(a, b)
+    assert a.sum() == k.sum()
+
+
+def test_epoch_sigma_clamp_is_kmeans_step():
+    rng = np.random.default_rng(7)
+    X = rng.random((2000, 3), np.float32)
+    W0 = rng.random((9, 3), np.float32)
+    W, *_ = R.train(W0, 3, 3, X, n_epochs=1, sigma=1e-20, topology="square", use_best_epoch=False, mode=FP64)
+    bmu = R.find_bmus(W0, X, FP64)
+    for j in range(9):
+        sel = X[bmu == j].astype(np.float64)
+        exp = sel.mean(0) if len(sel) else np.zeros(3)
+        assert np.allclose(W[j], exp, rtol=1e-6, atol=1e-7)
+
+
+def test_epoch_sigma_huge_is_global_mean():
+    rng = np.random.default_rng(8)
+    X = rng.random((1500, 4), np.float32)
+    W0 = rng.random((6, 4), np.float32)
+    W, *_ = R.train(W0, 2, 3, X, n_epochs=1, sigma=1e8, use_best_epoch=False, mode=FP64)
+    assert np.allclose(W, X.astype(np.float64).mean(0)[None, :], rtol=1e-6)
+
+
+def test_chunked_equals_single_chunk():
+    rng = np.random.default_rng(9)
+    X = rng.random((999, 4), np.float32)
+    rho = rng.random(999)
+    W0 = rng.random((16, 4), np.float32)
+
+    def gen(a, s):
+        return lambda: lambda: (a[i:i + s] for i in range(0, len(a), s))
+
+    a = R.train(W0, 4, 4, X, rho, n_epochs=2, mode=FP64)
+    b = R.train(W0, 4, 4, gen(X, 100), gen(rho, 100), n_epochs=2, mode=FP64)
+    assert np.allclose(a[0], b[0], rtol=1e-12)
+    assert np.allclose(a[2], b[2], rtol=1e-12)
+
+
+def test_factorised_fp64_matches_c_fp64():
+    rng = np.random.default_rng(10)
+    m, n, d = 5, 6, 7
+    X = rng.standard_normal((3000, d)).astype(np.float32)
+    rho = rng.random(3000)
+    W0 = rng.standard_normal((m * n, d)).astype(np.float32)
+    topo = R.topography(m, n, "hexagonal")
+    sigma = R.sigma_at(max(m, n) / 2.0, 0, 3)
+    Wf, c1, c2, d1 = R.factorised_epoch_fp64(W0, topo, X, rho, sigma)
+    num, den = R.epoch_accumulate(W0, topo, X, rho, 0, 3, np.float32(max(m, n) / 2.0), FP64)
+    Wc = R.divide(num, den, FP64)
+    assert np.allclose(Wf, Wc, rtol=1e-6, atol=1e-7)
+    g1, g2 = R.top2(W0, topo, X, None, FP64)
+    assert np.array_equal(c1, g1) and np.array_equal(c2, g2)
+
+
+def _z_reference_formula(nside, pix):
+    """decontamination/hp/__init__.py:120-131 (nest2xyf) and :376-419 (_xyf2loc, u = v = 0.5), restated."""
+    f = pix // (nside * nside)
+    v = pix & (nside * nside - 1)
+
+    def compress(b):
+        out = np.zeros_like(b)
+        for bit in range(32):
+            out |= ((b >> (2 * bit)) & 1) << bit
+        return out
+
+    x = (compress(v) + 0.5) / nside
+    y = (compress(v >> 1) + 0.5) / nside
+    r = 1.0 - np.floor_divide(f, 4.0)
+    h = r - 1.0 + (x + y)
+    mm = 2.0 - r * h
+    z = np.where(mm > 1.0, h * 2.0 / 3.0, r * (1.0 - mm ** 2 / 3.0))
+    return z
+
+
+@pytest.mark.parametrize("nside", [1, 2, 4, 16, 64])
+def test_healpix_pixel_centre_z(nside):
+    from decontamination_amd.synth import nest_pixel_z
+    pix = np.arange(12 * nside * nside, dtype=np.int64)
+    z = nest_pixel_z(nside, pix)
+    assert np.allclose(z, _z_reference_formula(nside, pix), rtol=0, atol=1e-12)
+    # equal-area pixels: z is symmetric about the equator and uniformly spread
+    assert abs(z.sum()) < 1e-9 * len(z)
+    assert np.all(np.abs(z) < 1.0)
