@@ -44,6 +44,9 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--precisions", default="fp32,bf16_rerank")
+    ap.add_argument("--serve-seconds", type=float, default=20.0)
+    ap.add_argument("--serve-rate", type=float, default=200.0)
+    ap.add_argument("--serve-budget-gb", type=float, default=64.0)
     return ap.parse_args()
 
 
@@ -143,6 +146,111 @@ def accuracy(ctx, res, X, topo, m, prec, arith, n_sub=65536):
     return {"subsample": len(idx), "bmu_disagreement_vs_fp64": dis, "bmu_agreement_vs_fp64": 1.0 - dis}
 
 
+def serving_bench(X16, seconds, rate, budget_gb, seed=0):
+    """Three resident trained SOMs (32x32/D16, 64x64/D16, 100x100/D32); BMU and activation-map requests
+    with log-uniform row counts in [1e3, 4e6], Poisson arrivals at `rate`/s for `seconds`."""
+    import threading
+    import decontamination_amd as dam
+    from decontamination_amd import synth
+    from decontamination_amd.serving import SOMServer, pinned_empty
+
+    t0 = time.perf_counter()
+    rng = np.random.default_rng(seed)
+    X32 = synth.catalogue_rows(4_200_000, 16, 14, seed=seed + 1)
+    specs = [(32, X16), (64, X16), (100, X32)]
+    soms = []
+    for m, data in specs:
+        som = dam.SOM_Batch(m, m, data.shape[1])
+        som.init_rand(seed=0)
+        som.train(data[: 1 << 20], n_epochs=4)
+        soms.append(som)
+    pools = {16: pinned_empty(X16.shape, np.float32), 32: pinned_empty(X32.shape, np.float32)}
+    pools[16][...] = X16
+    pools[32][...] = X32
+    outs = [pinned_empty((4_000_000,), np.int32) for _ in range(48)]
+    setup_s = time.perf_counter() - t0
+    log(f"serving setup {setup_s:.1f}s (3 models trained, pools pinned)")
+
+    # schedule
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, int(rate * seconds * 1.5) + 16))
+    arrivals = arrivals[arrivals < seconds]
+    nreq = len(arrivals)
+    models = rng.integers(0, 3, nreq)
+    kinds = rng.integers(0, 2, nreq)
+    rows = np.exp(rng.uniform(np.log(1e3), np.log(4e6), nreq)).astype(np.int64)
+    server = SOMServer(soms, memory_budget=int(budget_gb * (1 << 30)))
+    tickets = []
+    done = []
+    lock = threading.Lock()
+    stop = threading.Event()
+
+    def collector():
+        i = 0
+        while True:
+            with lock:
+                have = i < len(tickets)
+                t = tickets[i] if have else None
+            if not have:
+                if stop.is_set():
+                    return
+                time.sleep(0.0005)
+                continue
+            res, lat = server.wait(t[0])
+            done.append((lat, t[1], t[2], t[3]))
+            i += 1
+
+    th = threading.Thread(target=collector, daemon=True)
+    th.start()
+    start = time.perf_counter()
+    for k in range(nreq):
+        now = time.perf_counter() - start
+        if arrivals[k] > now:
+            time.sleep(arrivals[k] - now)
+        mi = int(models[k])
+        d = soms[mi].dim
+        pool = pools[d]
+        r = int(min(rows[k], pool.shape[0]))
+        off = int(rng.integers(0, pool.shape[0] - r + 1))
+        x = pool[off:off + r]
+        kind = "winners" if kinds[k] == 0 else "activation"
+        out = outs[k % len(outs)][:r] if kind == "winners" else None
+        t = server.submit(mi, kind, x, None, out)
+        with lock:
+            tickets.append((t, mi, kind, r))
+    stop.set()
+    th.join()
+    wall = time.perf_counter() - start
+    st = server.stats()
+    server.close()
+    lat = np.array([d[0] for d in done])
+    tot_rows = int(sum(d[3] for d in done))
+    by_model = {}
+    for mi, (m, data) in enumerate(specs):
+        l = np.array([d[0] for d in done if d[1] == mi])
+        if len(l):
+            by_model[f"{m}x{m}_d{data.shape[1]}"] = {"p50_ms": round(float(np.percentile(l, 50)), 3),
+                                                      "p99_ms": round(float(np.percentile(l, 99)), 3),
+                                                      "requests": int(len(l))}
+    return {
+        "p50_ms": round(float(np.percentile(lat, 50)), 3),
+        "p99_ms": round(float(np.percentile(lat, 99)), 3),
+        "mean_ms": round(float(lat.mean()), 3),
+        "max_ms": round(float(lat.max()), 3),
+        "requests": int(len(lat)),
+        "rate_per_s": rate,
+        "seconds": seconds,
+        "rows_per_s": round(tot_rows / wall, 1),
+        "offered_rows_per_s": round(float(rows.clip(max=4_000_000).sum()) / seconds, 1),
+        "launches": st["launches"],
+        "tiles": st["tiles"],
+        "budget_gb": budget_gb,
+        "by_model": by_model,
+        "workload": "3 resident SOMs (32x32/D16, 64x64/D16, 100x100/D32, trained 4 epochs on 1M rows), BMU or "
+                    "activation-map requests (50/50), rows log-uniform in [1e3, 4e6] sliced from pinned pools of "
+                    "synthetic catalogues, Poisson arrivals; latency = submit to result in host memory",
+    }
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -221,6 +329,10 @@ def main():
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(X, W0, topo, sigma0, E)
     out["serving"] = None
+    if args.serve_seconds > 0:
+        del x_dev
+        out["serving"] = serving_bench(X, args.serve_seconds, args.serve_rate, args.serve_budget_gb)
+        log(f"serving: {json.dumps(out['serving'])}")
     print(json.dumps(out), flush=True)
 
 

@@ -30,6 +30,9 @@ PREC_BF16_RERANK = 2
 RERANK_REF32 = 0
 RERANK_FP64 = 1
 
+SERVE_WINNERS = 0
+SERVE_ACTIVATION = 1
+
 TIMERS = {"bmu_exact": 0, "bmu_fp32": 1, "bmu_bf16": 2, "bmu_fallback": 3, "prep": 4, "errors": 5,
           "absmax": 6, "scatter": 7, "update": 8, "iter": 9}
 
@@ -74,6 +77,16 @@ SIGNATURES = {
     "somhip_batch_epoch_update": [P, I64, I64, F, P],
     "somhip_batch_iter_accumulate": [P, P, P, I64, I64, I64, F, I, I],
     "somhip_batch_iter_update": [P, P],
+    "somhip_host_alloc": [SZ, PP],
+    "somhip_host_free": [P],
+    "somhip_server_create": [P, SZ, I, PP],
+    "somhip_server_destroy": [P],
+    "somhip_server_add_model": [P, P, ctypes.POINTER(ctypes.c_int)],
+    "somhip_server_submit": [P, I, I, P, P, I64, P, ctypes.POINTER(ctypes.c_int64)],
+    "somhip_server_wait": [P, I64, ctypes.c_double, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double)],
+    "somhip_server_stats": [P, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64),
+                            ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64),
+                            ctypes.POINTER(ctypes.c_size_t)],
 }
 STRING_FUNCS = ("somhip_last_error", "somhip_version")
 

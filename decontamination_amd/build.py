@@ -21,7 +21,7 @@ ARCH = "gfx950"
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-# Packed FP32 VALU forms are disabled for every kernel; the rest is plain -O3 for gfx950.
+# Packed FP32 VALU forms are disabled for every kernel.
 FLAGS = [
     f"--offload-arch={ARCH}",
     "-O3",
@@ -30,6 +30,9 @@ FLAGS = [
     "-Wall",
     "-Wno-unused-function",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # MFMA results in VGPRs: the epilogues read every accumulator, and AGPRs would cost one
+    # v_accvgpr_read per value
+    "-mllvm", "-amdgpu-mfma-vgpr-form",
     "-I", os.path.join(ROOT, "include"),
 ]
 

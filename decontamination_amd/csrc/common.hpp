@@ -94,6 +94,36 @@ struct Top2 {
 };
 
 // ------------------------------------------------------------------------------------------------
+// Descriptors shared by the BMU kernels and the serving loop.
+// ------------------------------------------------------------------------------------------------
+// Everything the BF16 kernel needs to know about one prepared model.
+struct BmuModelDev {
+    const bf16x8 *frag;     // bf16 A fragments (permuted slots)
+    const f32x4 *norms4;    // |w'|^2 per permuted slot
+    const int *perm;        // slot -> neuron
+    const float *w;         // FP32 weights (mn, dim)
+    const float *centre;    // weight centroid (dpad)
+    const float *wmax;      // max_j |w'_jk| (dpad)
+    const float *scal;      // [0] max_j |w'_j|^2
+    int nblk, mn, dim;
+};
+
+// One unit of serving work: rows [row0, row0 + SOM_TILE_ROWS) of one request.
+struct BmuTile {
+    const float *x;     // request samples (n, dim)
+    int64_t n;          // request rows (0 = empty tile)
+    int64_t row0;       // first row of this tile
+    int32_t *c1;        // request outputs
+    int32_t *c2;
+    double *d1;
+    int model;          // index into the model table
+    int default_index;
+};
+
+constexpr int SOM_TILE_ROWS = 256;     // rows per BF16 workgroup (4 waves x 2 groups x 32)
+constexpr int FB_TILE_SHIFT = 40;      // serving fallback entry = (tile << 40) | row
+
+// ------------------------------------------------------------------------------------------------
 // Host-side structures.
 // ------------------------------------------------------------------------------------------------
 void somhip_set_error(const std::string &msg);
@@ -202,3 +232,9 @@ int somhip_timer_collect(somhip_ctx *C);
 int somhip_prepare(somhip_model *mdl);
 int somhip_launch_bmu(somhip_model *mdl, const float *x, int64_t n, int precision, int arithmetic, int need_c2,
                       int default_index, int32_t *c1, int32_t *c2, double *d1);
+BmuModelDev somhip_model_dev(const somhip_model *M);
+// Serving form of the BF16 search: `grid` tiles from `tiles` (n == 0 tiles are skipped), then the exact
+// fallback over the flagged (tile, row) entries with a fixed grid; fb_count must be zero on entry.
+int somhip_launch_bmu_tiles(hipStream_t st, int dpad, int arithmetic, int need_c2, const BmuModelDev *models,
+                            const BmuTile *tiles, int grid, int64_t *fb_list, unsigned int *fb_count);
+int somhip_launch_zero_u32(hipStream_t st, unsigned int *p);

@@ -150,16 +150,23 @@ __global__ __launch_bounds__(BMU_WAVES * WAVE) void bmu_fp32_kernel(
 #pragma unroll
     for (int g = 0; g < G; g++) { m1[g] = m2[g] = __builtin_inff(); i1[g] = i2[g] = -1; }
 
-    for (int blk = 0; blk < nblk; blk++) {
-        f32x4 a4[KS / 4];
+    auto load_block = [&](int blk, f32x4 (&a4)[KS / 4], f32x16 &cin) {
 #pragma unroll
         for (int q = 0; q < KS / 4; q++) a4[q] = frag[((int64_t)blk * (KS / 4) + q) * WAVE + lane];
-        f32x16 cin;
 #pragma unroll
         for (int q = 0; q < 4; q++) {
             f32x4 t = norms4[blk * 8 + 2 * q + khalf];
             cin[4 * q + 0] = t[0]; cin[4 * q + 1] = t[1]; cin[4 * q + 2] = t[2]; cin[4 * q + 3] = t[3];
         }
+    };
+    f32x4 a4[KS / 4];
+    f32x16 cin;
+    if (nblk > 0) load_block(0, a4, cin);
+    for (int blk = 0; blk < nblk; blk++) {
+        // prefetch the next block's fragments while this one computes
+        f32x4 na4[KS / 4];
+        f32x16 ncin;
+        if (blk + 1 < nblk) load_block(blk + 1, na4, ncin);
         const int jbase = blk * 32 + 4 * khalf;
 #pragma unroll
         for (int g = 0; g < G; g++) {
@@ -167,17 +174,25 @@ __global__ __launch_bounds__(BMU_WAVES * WAVE) void bmu_fp32_kernel(
 #pragma unroll
             for (int st = 0; st < KS; st++)
                 acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a4[st / 4][st % 4], b[g][st], acc, 0, 0, 0);
+            // in-order top-2 with strict '<' (lowest index wins ties within the lane):
+            // values by min/med3, indices by selects
 #pragma unroll
             for (int r = 0; r < 16; r++) {
                 const float v = acc[r];
                 const int j = jbase + 8 * (r >> 2) + (r & 3);
                 const bool lt1 = v < m1[g];
                 const bool lt2 = v < m2[g];
-                m2[g] = lt1 ? m1[g] : (lt2 ? v : m2[g]);
-                i2[g] = lt1 ? i1[g] : (lt2 ? j : i2[g]);
-                m1[g] = lt1 ? v : m1[g];
+                const int t2 = lt2 ? j : i2[g];
+                i2[g] = lt1 ? i1[g] : t2;
                 i1[g] = lt1 ? j : i1[g];
+                m2[g] = __builtin_amdgcn_fmed3f(m1[g], m2[g], v);
+                m1[g] = fminf(m1[g], v);
             }
+        }
+        if (blk + 1 < nblk) {
+#pragma unroll
+            for (int q = 0; q < KS / 4; q++) a4[q] = na4[q];
+            cin = ncin;
         }
     }
 
@@ -214,18 +229,23 @@ __global__ __launch_bounds__(BMU_WAVES * WAVE) void bmu_fp32_kernel(
 // BF16 3-term split with candidate window and exact rerank.
 // ------------------------------------------------------------------------------------------------
 template <int DP, int G, int MODE, bool NEED_C2>
-__global__ __launch_bounds__(BMU_WAVES * WAVE) void bmu_bf16_kernel(
-    const float *__restrict__ x, int64_t n, int dim, const float *__restrict__ centre,
-    const float *__restrict__ wmax, const float *__restrict__ scal, const bf16x8 *__restrict__ frag,
-    const f32x4 *__restrict__ norms4, const int *__restrict__ perm, int nblk, const float *__restrict__ w, int mn,
-    int default_index, int32_t *__restrict__ c1, int32_t *__restrict__ c2, double *__restrict__ d1,
-    int64_t *__restrict__ fb_list, unsigned int *__restrict__ fb_count)
+__device__ __forceinline__ void bmu_bf16_body(const BmuModelDev &Md, const float *__restrict__ x, int64_t n,
+                                              int64_t sbase, int default_index, int32_t *__restrict__ c1,
+                                              int32_t *__restrict__ c2, double *__restrict__ d1,
+                                              int64_t *__restrict__ fb_list, unsigned int *__restrict__ fb_count,
+                                              int64_t fb_tag, unsigned short (*cw)[CAPL][WAVE])
 {
     constexpr int KC = DP / 16;
-    __shared__ unsigned short cand[BMU_WAVES][G][CAPL][WAVE];
+    const float *__restrict__ centre = Md.centre;
+    const float *__restrict__ wmax = Md.wmax;
+    const float *__restrict__ scal = Md.scal;
+    const bf16x8 *__restrict__ frag = Md.frag;
+    const f32x4 *__restrict__ norms4 = Md.norms4;
+    const int *__restrict__ perm = Md.perm;
+    const float *__restrict__ w = Md.w;
+    const int nblk = Md.nblk, mn = Md.mn, dim = Md.dim;
+    (void)mn;
     const int lane = threadIdx.x & (WAVE - 1);
-    const int wave = threadIdx.x / WAVE;
-    const int64_t sbase = ((int64_t)blockIdx.x * BMU_WAVES + wave) * (G * 32);
     const int col = lane & 31;
     const int khalf = lane >> 5;
 
@@ -293,10 +313,13 @@ __global__ __launch_bounds__(BMU_WAVES * WAVE) void bmu_bf16_kernel(
         }
     };
 
+    bf16x8 ah[KC], al[KC];
+    f32x16 cin;
+    if (nblk > 0) load_block(0, ah, al, cin);
     for (int blk = 0; blk < nblk; blk++) {
-        bf16x8 ah[KC], al[KC];
-        f32x16 cin;
-        load_block(blk, ah, al, cin);
+        bf16x8 nah[KC], nal[KC];
+        f32x16 ncin;
+        if (blk + 1 < nblk) load_block(blk + 1, nah, nal, ncin);
 #pragma unroll
         for (int g = 0; g < G; g++) {
             f32x16 acc = block_acc(blk, g, ah, al, cin);
@@ -305,6 +328,11 @@ __global__ __launch_bounds__(BMU_WAVES * WAVE) void bmu_bf16_kernel(
             for (int r = 0; r < 16; r += 2) t = fminf(fminf(t, acc[r]), acc[r + 1]);
             m2[g] = __builtin_amdgcn_fmed3f(m1[g], m2[g], t);
             m1[g] = fminf(m1[g], t);
+        }
+        if (blk + 1 < nblk) {
+#pragma unroll
+            for (int kc = 0; kc < KC; kc++) { ah[kc] = nah[kc]; al[kc] = nal[kc]; }
+            cin = ncin;
         }
     }
 
@@ -324,21 +352,34 @@ __global__ __launch_bounds__(BMU_WAVES * WAVE) void bmu_bf16_kernel(
     int cnt[G];
 #pragma unroll
     for (int g = 0; g < G; g++) cnt[g] = 0;
+    if (nblk > 0) load_block(0, ah, al, cin);
     for (int blk = 0; blk < nblk; blk++) {
-        bf16x8 ah[KC], al[KC];
-        f32x16 cin;
-        load_block(blk, ah, al, cin);
+        bf16x8 nah[KC], nal[KC];
+        f32x16 ncin;
+        if (blk + 1 < nblk) load_block(blk + 1, nah, nal, ncin);
         const int sbase_blk = blk * 32 + 4 * khalf;
 #pragma unroll
         for (int g = 0; g < G; g++) {
             f32x16 acc = block_acc(blk, g, ah, al, cin);
+            // pre-check: skip the per-value compares when no lane of the wave has a hit
+            float t = __builtin_inff();
 #pragma unroll
-            for (int r = 0; r < 16; r++) {
-                if (acc[r] <= T[g]) {
-                    if (cnt[g] < CAPL) cand[wave][g][cnt[g]][lane] = (unsigned short)(sbase_blk + 8 * (r >> 2) + (r & 3));
-                    cnt[g]++;
+            for (int r = 0; r < 16; r += 2) t = fminf(fminf(t, acc[r]), acc[r + 1]);
+            if (__builtin_amdgcn_ballot_w64(t <= T[g]) != 0ull) {
+#pragma unroll
+                for (int r = 0; r < 16; r++) {
+                    if (acc[r] <= T[g]) {
+                        if (cnt[g] < CAPL)
+                            cw[g][cnt[g]][lane] = (unsigned short)(sbase_blk + 8 * (r >> 2) + (r & 3));
+                        cnt[g]++;
+                    }
                 }
             }
+        }
+        if (blk + 1 < nblk) {
+#pragma unroll
+            for (int kc = 0; kc < KC; kc++) { ah[kc] = nah[kc]; al[kc] = nal[kc]; }
+            cin = ncin;
         }
     }
 
@@ -356,7 +397,7 @@ __global__ __launch_bounds__(BMU_WAVES * WAVE) void bmu_bf16_kernel(
 #pragma unroll
             for (int k = 0; k < DP; k++) xr[k] = k < dim ? x[s * dim + k] : 0.f;
             for (int e = 0; e < mycnt; e++) {
-                const int slot = cand[wave][g][e][lane];
+                const int slot = cw[g][e][lane];
                 const int j = perm[slot];
                 if (j < 0) continue;
                 float wr[DP];
@@ -375,13 +416,83 @@ __global__ __launch_bounds__(BMU_WAVES * WAVE) void bmu_bf16_kernel(
         if (lane < 32 && valid_s) {
             if (bad) {
                 const unsigned int slot = atomicAdd(fb_count, 1u);
-                fb_list[slot] = s;
+                fb_list[slot] = fb_tag | s;
             } else {
                 c1[s] = tp.i1 >= 0 ? tp.i1 : default_index;
                 if (c2) c2[s] = tp.i2;
                 if (d1) d1[s] = tp.i1 >= 0 ? tp.d1 : __builtin_nan("");
             }
         }
+    }
+}
+
+template <int DP, int G, int MODE, bool NEED_C2>
+__global__ __launch_bounds__(BMU_WAVES * WAVE) void bmu_bf16_kernel(BmuModelDev Md, const float *__restrict__ x,
+                                                                   int64_t n, int default_index,
+                                                                   int32_t *__restrict__ c1, int32_t *__restrict__ c2,
+                                                                   double *__restrict__ d1,
+                                                                   int64_t *__restrict__ fb_list,
+                                                                   unsigned int *__restrict__ fb_count)
+{
+    __shared__ unsigned short cand[BMU_WAVES][G][CAPL][WAVE];
+    const int wave = threadIdx.x / WAVE;
+    const int64_t sbase = ((int64_t)blockIdx.x * BMU_WAVES + wave) * (G * 32);
+    bmu_bf16_body<DP, G, MODE, NEED_C2>(Md, x, n, sbase, default_index, c1, c2, d1, fb_list, fb_count, 0,
+                                        cand[wave]);
+}
+
+// Serving form: one workgroup per tile; tiles of different requests and models share one launch.
+template <int DP, int G, int MODE, bool NEED_C2>
+__global__ __launch_bounds__(BMU_WAVES * WAVE) void bmu_bf16_tiles_kernel(const BmuModelDev *__restrict__ models,
+                                                                         const BmuTile *__restrict__ tiles,
+                                                                         int64_t *__restrict__ fb_list,
+                                                                         unsigned int *__restrict__ fb_count)
+{
+    __shared__ unsigned short cand[BMU_WAVES][G][CAPL][WAVE];
+    const BmuTile T = tiles[blockIdx.x];
+    if (T.n <= 0) return;
+    const BmuModelDev Md = models[T.model];
+    const int wave = threadIdx.x / WAVE;
+    const int64_t sbase = T.row0 + (int64_t)wave * (G * 32);
+    bmu_bf16_body<DP, G, MODE, NEED_C2>(Md, T.x, T.n, sbase, T.default_index, T.c1, T.c2, T.d1, fb_list, fb_count,
+                                        (int64_t)blockIdx.x << FB_TILE_SHIFT, cand[wave]);
+}
+
+// Fallback of the serving form: exact scan for (tile, row) entries, weights read from global memory
+// (entries of one launch may belong to different models).
+template <int DP, int MODE, bool NEED_C2>
+__global__ __launch_bounds__(256) void bmu_exact_tiles_kernel(const BmuModelDev *__restrict__ models,
+                                                             const BmuTile *__restrict__ tiles,
+                                                             const int64_t *__restrict__ list,
+                                                             const unsigned int *__restrict__ list_count)
+{
+    const int64_t total = (int64_t)(*list_count);
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t e = list[t];
+        const BmuTile T = tiles[e >> FB_TILE_SHIFT];
+        const int64_t i = e & ((1ll << FB_TILE_SHIFT) - 1);
+        const BmuModelDev Md = models[T.model];
+        const int dim = Md.dim;
+        float xr[DP];
+#pragma unroll
+        for (int k = 0; k < DP; k++) xr[k] = k < dim ? T.x[i * dim + k] : 0.f;
+        double md1 = 1.0e99, md2 = 1.0e99;
+        int mi1 = -1, mi2 = -1;
+        for (int j = 0; j < Md.mn; j++) {
+            float wr[DP];
+#pragma unroll
+            for (int k = 0; k < DP; k++) wr[k] = k < dim ? Md.w[(int64_t)j * dim + k] : 0.f;
+            const double d = sqd_mode<MODE>(wr, xr, DP);
+            if (d != d) continue;
+            if (md1 > d) {
+                md2 = md1; mi2 = mi1; md1 = d; mi1 = j;
+            } else if (NEED_C2 && md2 > d) {
+                md2 = d; mi2 = j;
+            }
+        }
+        T.c1[i] = mi1 >= 0 ? mi1 : T.default_index;
+        if (T.c2) T.c2[i] = mi2;
+        if (T.d1) T.d1[i] = mi1 >= 0 ? md1 : __builtin_nan("");
     }
 }
 
@@ -474,6 +585,9 @@ void launch_exact(int mode, bool need_c2, hipStream_t st, const float *x, int64_
 
 constexpr int G_FP32 = 2;
 constexpr int G_BF16 = 2;
+static_assert(BMU_WAVES * G_BF16 * 32 == SOM_TILE_ROWS, "tile rows must match the BF16 workgroup");
+
+__global__ void zero_u32_kernel(unsigned int *p) { *p = 0u; }
 
 template <int DP>
 void launch_fp32(hipStream_t st, somhip_model *M, const float *x, int64_t n, int default_index, int32_t *c1,
@@ -494,9 +608,8 @@ void launch_bf16(int mode, bool need_c2, hipStream_t st, somhip_model *M, const 
     const int64_t blocks = (n + per_block - 1) / per_block;
     dim3 grid((unsigned)blocks), block(BMU_WAVES * WAVE);
 #define SOM_BF(MO, C)                                                                                           \
-    hipLaunchKernelGGL((bmu_bf16_kernel<DP, G_BF16, MO, C>), grid, block, 0, st, x, n, M->dim, M->centre, M->wmax, \
-                       M->scal, (const bf16x8 *)M->frag_bf, (const f32x4 *)M->norms_perm, M->perm, M->nblk, M->w,   \
-                       M->mn, default_index, c1, c2, d1, fb_list, fb_count)
+    hipLaunchKernelGGL((bmu_bf16_kernel<DP, G_BF16, MO, C>), grid, block, 0, st, somhip_model_dev(M), x, n, default_index, c1, \
+                       c2, d1, fb_list, fb_count)
     if (mode == SOMHIP_RERANK_FP64) {
         if (need_c2) SOM_BF(SOMHIP_RERANK_FP64, true); else SOM_BF(SOMHIP_RERANK_FP64, false);
     } else {
@@ -509,6 +622,62 @@ void launch_bf16(int mode, bool need_c2, hipStream_t st, somhip_model *M, const 
 
 // ------------------------------------------------------------------------------------------------
 // Internal entry points.
+// ------------------------------------------------------------------------------------------------
+BmuModelDev somhip_model_dev(const somhip_model *M)
+{
+    BmuModelDev d;
+    d.frag = (const bf16x8 *)M->frag_bf;
+    d.norms4 = (const f32x4 *)M->norms_perm;
+    d.perm = M->perm;
+    d.w = M->w;
+    d.centre = M->centre;
+    d.wmax = M->wmax;
+    d.scal = M->scal;
+    d.nblk = M->nblk;
+    d.mn = M->mn;
+    d.dim = M->dim;
+    return d;
+}
+
+int somhip_launch_zero_u32(hipStream_t st, unsigned int *p)
+{
+    hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(1), 0, st, p);
+    SOM_HIP_TRY(hipGetLastError());
+    return SOMHIP_OK;
+}
+
+int somhip_launch_bmu_tiles(hipStream_t st, int dpad, int arithmetic, int need_c2, const BmuModelDev *models,
+                            const BmuTile *tiles, int grid, int64_t *fb_list, unsigned int *fb_count)
+{
+    if (grid <= 0) return SOMHIP_OK;
+    if (dpad != 16 && dpad != 32) {
+        somhip_set_error("serving supports dim <= 32");
+        return SOMHIP_ERR_UNSUPPORTED;
+    }
+    dim3 g((unsigned)grid), b(BMU_WAVES * WAVE);
+    const int fb_grid = std::max(1, std::min(grid, 1024));
+#define SOM_TL(D, MO, C)                                                                                  \
+    do {                                                                                                  \
+        hipLaunchKernelGGL((bmu_bf16_tiles_kernel<D, G_BF16, MO, C>), g, b, 0, st, models, tiles, fb_list, \
+                           fb_count);                                                                    \
+        hipLaunchKernelGGL((bmu_exact_tiles_kernel<D, MO, C>), dim3(fb_grid), dim3(256), 0, st, models,    \
+                           tiles, fb_list, fb_count);                                                    \
+    } while (0)
+#define SOM_TL2(D)                                                                                        \
+    do {                                                                                                  \
+        if (arithmetic == SOMHIP_RERANK_FP64) {                                                           \
+            if (need_c2) SOM_TL(D, SOMHIP_RERANK_FP64, true); else SOM_TL(D, SOMHIP_RERANK_FP64, false);  \
+        } else {                                                                                          \
+            if (need_c2) SOM_TL(D, SOMHIP_RERANK_REF32, true); else SOM_TL(D, SOMHIP_RERANK_REF32, false); \
+        }                                                                                                 \
+    } while (0)
+    if (dpad == 16) SOM_TL2(16);
+    else SOM_TL2(32);
+#undef SOM_TL2
+#undef SOM_TL
+    SOM_HIP_TRY(hipGetLastError());
+    return SOMHIP_OK;
+}
 // ------------------------------------------------------------------------------------------------
 int somhip_prepare(somhip_model *M)
 {

@@ -1,0 +1,611 @@
+// Serving loop: several trained SOMs resident in HBM, BMU and activation-map requests of uneven
+// sizes batched continuously into shared launches.
+//
+//   submit()  -> pending queue (FIFO)
+//   admission -> a request is admitted when its working memory fits in the free part of the budget
+//                (one device slab, first-fit); its rows are uploaded on the copy stream
+//   batching  -> every iteration, the tiles (256 rows) of all uploaded requests are shared out
+//                round-robin into one launch per dimension class; the launch is a hipGraph captured
+//                per (dimension, tile-count bucket) with a fixed tile-table buffer, so a replay is a
+//                table upload plus one graph launch
+//   finishing -> a request whose last tile has been launched gets its activation reduction and the
+//                copy of its result to the host enqueued; its completion event frees its memory
+//
+// Requests never share accumulators: BMUs are per row, activation maps are exact fixed-point sums
+// scaled from the request's own size, so a result is bit-identical alone and in any mix.
+#include "common.hpp"
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+// launchers from k_train.hip
+int somhip_launch_absmax(hipStream_t st, const float *x, int dim, int dpad, const double *rho, int rho_f32, int64_t n,
+                         unsigned int *bounds, int num_cus);
+int somhip_launch_shifts(hipStream_t st, const unsigned int *bounds, int dim, int has_x, int has_rho, int64_t n,
+                         int *shifts);
+int somhip_scatter_nwg(int64_t n, int mn, int nv, int num_cus, size_t table_budget);
+int somhip_launch_scatter(hipStream_t st, const int32_t *bmu, const float *x, int dim, const double *rho, int rho_f32,
+                          int64_t n, const int *shifts, int mn, int nv, int nwg, unsigned long long *partial,
+                          double *acc, size_t lds_limit);
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+constexpr int MAX_MODELS = 64;
+constexpr int BUCKETS[] = {16, 64, 256, 1024, 4096};
+constexpr int NBUCKETS = sizeof(BUCKETS) / sizeof(BUCKETS[0]);
+constexpr int MAX_TILES = 4096;
+constexpr int STAGING_SLOTS = 8;
+constexpr size_t ALIGN = 256;
+constexpr size_t ACT_TABLE_BUDGET = (size_t)32 << 20;
+
+inline size_t align_up(size_t v) { return (v + ALIGN - 1) / ALIGN * ALIGN; }
+
+// First-fit allocator over one device slab (offsets only).
+struct Slab {
+    char *base = nullptr;
+    size_t capacity = 0, used = 0;
+    std::map<size_t, size_t> free_;  // offset -> size
+    void init(char *b, size_t cap)
+    {
+        base = b;
+        capacity = cap;
+        free_.clear();
+        free_[0] = cap;
+        used = 0;
+    }
+    bool alloc(size_t bytes, size_t *off)
+    {
+        bytes = align_up(bytes);
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second >= bytes) {
+                *off = it->first;
+                size_t rest = it->second - bytes;
+                size_t o = it->first;
+                free_.erase(it);
+                if (rest) free_[o + bytes] = rest;
+                used += bytes;
+                return true;
+            }
+        }
+        return false;
+    }
+    void release(size_t off, size_t bytes)
+    {
+        bytes = align_up(bytes);
+        used -= bytes;
+        auto it = free_.emplace(off, bytes).first;
+        auto nx = std::next(it);
+        if (nx != free_.end() && it->first + it->second == nx->first) {
+            it->second += nx->second;
+            free_.erase(nx);
+        }
+        if (it != free_.begin()) {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second == it->first) {
+                pv->second += it->second;
+                free_.erase(it);
+            }
+        }
+    }
+};
+
+enum State { QUEUED = 0, UPLOADING = 1, READY = 2, FINISHING = 3, DONE = 4, FAILED = 5 };
+
+struct Request {
+    int64_t ticket = 0;
+    int model = 0, kind = 0;
+    const float *x_host = nullptr;
+    const double *rho_host = nullptr;
+    int64_t n = 0;
+    void *out_host = nullptr;
+    State state = QUEUED;
+    size_t off = 0, bytes = 0;
+    float *x = nullptr;
+    double *rho = nullptr;
+    int32_t *c1 = nullptr;
+    double *act = nullptr;
+    unsigned long long *table = nullptr;
+    unsigned int *bounds = nullptr;
+    int *shifts = nullptr;
+    int nwg = 1;
+    int64_t next_row = 0;
+    hipEvent_t ev_up = nullptr, ev_done = nullptr;
+    Clock::time_point t_submit, t_done;
+};
+
+struct GraphSlot {
+    bool built = false;
+    hipGraphExec_t exec = nullptr;
+    BmuTile *tiles = nullptr;       // device table baked into the graph
+    int64_t *fb_list = nullptr;     // fallback entries
+    unsigned int *fb_count = nullptr;
+};
+
+}  // namespace
+
+struct somhip_server {
+    somhip_ctx *ctx = nullptr;
+    int arithmetic = SOMHIP_RERANK_REF32;
+    hipStream_t compute = nullptr, copy = nullptr;
+    char *slab_mem = nullptr;
+    Slab slab;
+    std::vector<somhip_model *> models;
+    std::vector<int> model_dpad;
+    BmuModelDev *models_dev = nullptr;
+    GraphSlot graphs[2][NBUCKETS];
+    BmuTile *staging = nullptr;  // pinned host, STAGING_SLOTS * MAX_TILES
+    hipEvent_t staging_ev[STAGING_SLOTS] = {};
+    int staging_next = 0;
+
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<std::shared_ptr<Request>> pending;
+    std::unordered_map<int64_t, std::shared_ptr<Request>> all;
+    std::vector<std::shared_ptr<Request>> active;  // scheduler-thread owned
+    int64_t next_ticket = 1;
+    bool stop = false;
+    std::string error;
+    std::thread worker;
+
+    // statistics
+    std::atomic<int64_t> completed{0}, launches{0}, tiles_launched{0};
+
+    int build_graph(int dclass, int b);
+    int launch_tiles(int dclass, const std::vector<BmuTile> &tl);
+    int finish(Request &R);
+    void loop();
+    void fail(const std::string &msg)
+    {
+        std::lock_guard<std::mutex> g(mu);
+        if (error.empty()) error = msg;
+        for (auto &kv : all)
+            if (kv.second->state != DONE) kv.second->state = FAILED;
+        cv_done.notify_all();
+    }
+};
+
+namespace {
+int dclass_of(int dpad) { return dpad == 16 ? 0 : 1; }
+}  // namespace
+
+int somhip_server::build_graph(int dc, int b)
+{
+    GraphSlot &G = graphs[dc][b];
+    const int grid = BUCKETS[b];
+    SOM_HIP_TRY(hipMalloc((void **)&G.tiles, sizeof(BmuTile) * grid));
+    SOM_HIP_TRY(hipMalloc((void **)&G.fb_list, sizeof(int64_t) * (size_t)grid * SOM_TILE_ROWS));
+    SOM_HIP_TRY(hipMalloc((void **)&G.fb_count, sizeof(unsigned int)));
+    // empty tiles are skipped by the kernels
+    std::vector<BmuTile> empty(grid);
+    for (auto &t : empty) t = BmuTile{};
+    SOM_HIP_TRY(hipMemcpy(G.tiles, empty.data(), sizeof(BmuTile) * grid, hipMemcpyHostToDevice));
+    hipGraph_t graph = nullptr;
+    SOM_HIP_TRY(hipStreamBeginCapture(compute, hipStreamCaptureModeThreadLocal));
+    // the fallback counter is cleared by a kernel node (no memset node in the graph)
+    int rc = somhip_launch_zero_u32(compute, G.fb_count);
+    if (rc == SOMHIP_OK)
+        rc = somhip_launch_bmu_tiles(compute, dc == 0 ? 16 : 32, arithmetic, 0, models_dev, G.tiles, grid, G.fb_list,
+                                     G.fb_count);
+    hipError_t e = hipStreamEndCapture(compute, &graph);
+    if (rc != SOMHIP_OK) return rc;
+    SOM_HIP_TRY(e);
+    SOM_HIP_TRY(hipGraphInstantiate(&G.exec, graph, nullptr, nullptr, 0));
+    SOM_HIP_TRY(hipGraphDestroy(graph));
+    G.built = true;
+    return SOMHIP_OK;
+}
+
+int somhip_server::launch_tiles(int dc, const std::vector<BmuTile> &tl)
+{
+    const int nt = (int)tl.size();
+    int b = 0;
+    while (b < NBUCKETS - 1 && BUCKETS[b] < nt) b++;
+    GraphSlot &G = graphs[dc][b];
+    if (!G.built) {
+        int rc = build_graph(dc, b);
+        if (rc) return rc;
+    }
+    // stage the table in pinned memory; pad with empty tiles up to the bucket
+    const int slot = staging_next;
+    staging_next = (staging_next + 1) % STAGING_SLOTS;
+    SOM_HIP_TRY(hipEventSynchronize(staging_ev[slot]));
+    BmuTile *st = staging + (size_t)slot * MAX_TILES;
+    std::copy(tl.begin(), tl.end(), st);
+    for (int i = nt; i < BUCKETS[b]; i++) st[i] = BmuTile{};
+    SOM_HIP_TRY(hipMemcpyAsync(G.tiles, st, sizeof(BmuTile) * BUCKETS[b], hipMemcpyHostToDevice, compute));
+    SOM_HIP_TRY(hipEventRecord(staging_ev[slot], compute));
+    SOM_HIP_TRY(hipGraphLaunch(G.exec, compute));
+    launches++;
+    tiles_launched += nt;
+    return SOMHIP_OK;
+}
+
+int somhip_server::finish(Request &R)
+{
+    somhip_model *M = models[R.model];
+    if (R.kind == SOMHIP_SERVE_ACTIVATION) {
+        int rc = somhip_launch_absmax(compute, nullptr, 0, 16, R.rho, 0, R.rho ? R.n : 0, R.bounds, ctx->num_cus);
+        if (!rc) rc = somhip_launch_shifts(compute, R.bounds, 0, 0, R.rho != nullptr, R.n, R.shifts);
+        if (!rc) SOM_HIP_TRY(hipMemsetAsync(R.act, 0, sizeof(double) * M->mn, compute));
+        if (!rc)
+            rc = somhip_launch_scatter(compute, R.c1, nullptr, 0, R.rho, 0, R.n, R.shifts, M->mn, 1, R.nwg, R.table,
+                                       R.act, ctx->lds_per_block);
+        if (rc) return rc;
+        SOM_HIP_TRY(hipMemcpyAsync(R.out_host, R.act, sizeof(double) * M->mn, hipMemcpyDeviceToHost, compute));
+    } else if (R.n > 0) {
+        SOM_HIP_TRY(hipMemcpyAsync(R.out_host, R.c1, sizeof(int32_t) * R.n, hipMemcpyDeviceToHost, compute));
+    }
+    SOM_HIP_TRY(hipEventRecord(R.ev_done, compute));
+    R.state = FINISHING;
+    return SOMHIP_OK;
+}
+
+void somhip_server::loop()
+{
+    if (hipSetDevice(ctx->device) != hipSuccess) {
+        fail("hipSetDevice failed in the serving thread");
+        return;
+    }
+    for (;;) {
+        bool progress = false;
+        {
+            std::unique_lock<std::mutex> g(mu);
+            if (stop && pending.empty() && active.empty()) break;
+            if (!error.empty()) break;
+            // admission: FIFO while the head fits in the free budget
+            while (!pending.empty()) {
+                auto R = pending.front();
+                size_t off;
+                if (!slab.alloc(R->bytes, &off)) break;
+                pending.pop_front();
+                R->off = off;
+                char *p = slab.base + off;
+                somhip_model *M = models[R->model];
+                auto take = [&](size_t sz) { char *q = p; p += align_up(sz); return q; };
+                R->x = (float *)take(sizeof(float) * R->n * M->dim);
+                R->c1 = (int32_t *)take(sizeof(int32_t) * std::max<int64_t>(R->n, 1));
+                if (R->rho_host) R->rho = (double *)take(sizeof(double) * R->n);
+                if (R->kind == SOMHIP_SERVE_ACTIVATION) {
+                    R->act = (double *)take(sizeof(double) * M->mn);
+                    R->bounds = (unsigned int *)take(512);
+                    R->shifts = (int *)take(512);
+                    R->table = (unsigned long long *)take(sizeof(unsigned long long) * (size_t)R->nwg * M->mn);
+                }
+                active.push_back(R);
+                g.unlock();
+                hipError_t e = hipSuccess;
+                if (R->n > 0)
+                    e = hipMemcpyAsync(R->x, R->x_host, sizeof(float) * R->n * M->dim, hipMemcpyHostToDevice, copy);
+                if (e == hipSuccess && R->rho)
+                    e = hipMemcpyAsync(R->rho, R->rho_host, sizeof(double) * R->n, hipMemcpyHostToDevice, copy);
+                if (e == hipSuccess) e = hipEventRecord(R->ev_up, copy);
+                g.lock();
+                if (e != hipSuccess) {
+                    g.unlock();
+                    fail(std::string("upload: ") + hipGetErrorString(e));
+                    return;
+                }
+                R->state = UPLOADING;
+                progress = true;
+            }
+        }
+        // uploads that completed
+        for (auto &R : active) {
+            if (R->state == UPLOADING) {
+                hipError_t q = hipEventQuery(R->ev_up);
+                if (q == hipSuccess) {
+                    R->state = READY;
+                    progress = true;
+                } else if (q != hipErrorNotReady) {
+                    fail(std::string("upload event: ") + hipGetErrorString(q));
+                    return;
+                }
+            }
+        }
+        // batching: share tiles round-robin among the ready requests of each dimension class
+        for (int dc = 0; dc < 2; dc++) {
+            std::vector<Request *> ready;
+            for (auto &R : active)
+                if (R->state == READY && dclass_of(model_dpad[R->model]) == dc) ready.push_back(R.get());
+            if (ready.empty()) continue;
+            std::vector<BmuTile> tl;
+            tl.reserve(MAX_TILES);
+            std::vector<Request *> finished;
+            const int share = std::max(16, MAX_TILES / (int)ready.size());
+            for (int pass = 0; pass < 2 && (int)tl.size() < MAX_TILES; pass++) {
+                for (Request *R : ready) {
+                    int quota = pass == 0 ? share : MAX_TILES;
+                    while (R->next_row < R->n && quota > 0 && (int)tl.size() < MAX_TILES) {
+                        BmuTile t;
+                        t.x = R->x;
+                        t.n = R->n;
+                        t.row0 = R->next_row;
+                        t.c1 = R->c1;
+                        t.c2 = nullptr;
+                        t.d1 = nullptr;
+                        t.model = R->model;
+                        t.default_index = 0;
+                        tl.push_back(t);
+                        R->next_row += SOM_TILE_ROWS;
+                        quota--;
+                    }
+                }
+            }
+            for (Request *R : ready)
+                if (R->next_row >= R->n) finished.push_back(R);
+            if (!tl.empty()) {
+                int rc = launch_tiles(dc, tl);
+                if (rc) {
+                    fail(std::string("launch: ") + somhip_last_error());
+                    return;
+                }
+            }
+            for (Request *R : finished) {
+                int rc = finish(*R);
+                if (rc) {
+                    fail(std::string("finish: ") + somhip_last_error());
+                    return;
+                }
+            }
+            progress = true;
+        }
+        // completions
+        for (size_t i = 0; i < active.size();) {
+            auto &R = active[i];
+            if (R->state == FINISHING) {
+                hipError_t q = hipEventQuery(R->ev_done);
+                if (q == hipSuccess) {
+                    std::lock_guard<std::mutex> g(mu);
+                    R->t_done = Clock::now();
+                    R->state = DONE;
+                    slab.release(R->off, R->bytes);
+                    completed++;
+                    active.erase(active.begin() + i);
+                    cv_done.notify_all();
+                    progress = true;
+                    continue;
+                } else if (q != hipErrorNotReady) {
+                    fail(std::string("completion event: ") + hipGetErrorString(q));
+                    return;
+                }
+            }
+            i++;
+        }
+        if (!progress) {
+            std::unique_lock<std::mutex> g(mu);
+            if (pending.empty() || active.size() > 0)
+                cv_work.wait_for(g, std::chrono::microseconds(active.empty() ? 2000 : 20));
+        }
+    }
+}
+
+extern "C" {
+
+int somhip_host_alloc(size_t bytes, void **ptr)
+{
+    if (!ptr) return SOMHIP_ERR_ARG;
+    SOM_HIP_TRY(hipHostMalloc(ptr, bytes ? bytes : 1, hipHostMallocDefault));
+    return SOMHIP_OK;
+}
+
+int somhip_host_free(void *ptr)
+{
+    if (!ptr) return SOMHIP_OK;
+    SOM_HIP_TRY(hipHostFree(ptr));
+    return SOMHIP_OK;
+}
+
+int somhip_server_create(somhip_ctx *C, size_t memory_budget, int arithmetic, somhip_server **out)
+{
+    if (!C || !out) {
+        somhip_set_error("null argument");
+        return SOMHIP_ERR_ARG;
+    }
+    if (arithmetic != SOMHIP_RERANK_REF32 && arithmetic != SOMHIP_RERANK_FP64) {
+        somhip_set_error("invalid arithmetic");
+        return SOMHIP_ERR_ARG;
+    }
+    if (memory_budget < ((size_t)1 << 20)) {
+        somhip_set_error("memory budget below 1 MiB");
+        return SOMHIP_ERR_ARG;
+    }
+    *out = nullptr;
+    SOM_HIP_TRY(hipSetDevice(C->device));
+    auto S = std::make_unique<somhip_server>();
+    S->ctx = C;
+    S->arithmetic = arithmetic;
+    SOM_HIP_TRY(hipStreamCreateWithFlags(&S->compute, hipStreamNonBlocking));
+    SOM_HIP_TRY(hipStreamCreateWithFlags(&S->copy, hipStreamNonBlocking));
+    SOM_HIP_TRY(hipMalloc((void **)&S->slab_mem, memory_budget));
+    S->slab.init(S->slab_mem, memory_budget);
+    SOM_HIP_TRY(hipMalloc((void **)&S->models_dev, sizeof(BmuModelDev) * MAX_MODELS));
+    SOM_HIP_TRY(hipHostMalloc((void **)&S->staging, sizeof(BmuTile) * MAX_TILES * STAGING_SLOTS, hipHostMallocDefault));
+    for (int i = 0; i < STAGING_SLOTS; i++) {
+        SOM_HIP_TRY(hipEventCreateWithFlags(&S->staging_ev[i], hipEventDisableTiming));
+        SOM_HIP_TRY(hipEventRecord(S->staging_ev[i], S->compute));
+    }
+    somhip_server *raw = S.release();
+    raw->worker = std::thread([raw] { raw->loop(); });
+    *out = raw;
+    return SOMHIP_OK;
+}
+
+int somhip_server_destroy(somhip_server *S)
+{
+    if (!S) return SOMHIP_OK;
+    {
+        std::lock_guard<std::mutex> g(S->mu);
+        S->stop = true;
+    }
+    S->cv_work.notify_all();
+    if (S->worker.joinable()) S->worker.join();
+    (void)hipSetDevice(S->ctx->device);
+    (void)hipStreamSynchronize(S->compute);
+    (void)hipStreamSynchronize(S->copy);
+    for (auto &kv : S->all) {
+        if (kv.second->ev_up) (void)hipEventDestroy(kv.second->ev_up);
+        if (kv.second->ev_done) (void)hipEventDestroy(kv.second->ev_done);
+    }
+    for (int dc = 0; dc < 2; dc++)
+        for (int b = 0; b < NBUCKETS; b++) {
+            GraphSlot &G = S->graphs[dc][b];
+            if (G.exec) (void)hipGraphExecDestroy(G.exec);
+            if (G.tiles) (void)hipFree(G.tiles);
+            if (G.fb_list) (void)hipFree(G.fb_list);
+            if (G.fb_count) (void)hipFree(G.fb_count);
+        }
+    for (int i = 0; i < STAGING_SLOTS; i++)
+        if (S->staging_ev[i]) (void)hipEventDestroy(S->staging_ev[i]);
+    if (S->staging) (void)hipHostFree(S->staging);
+    if (S->models_dev) (void)hipFree(S->models_dev);
+    if (S->slab_mem) (void)hipFree(S->slab_mem);
+    (void)hipStreamDestroy(S->compute);
+    (void)hipStreamDestroy(S->copy);
+    delete S;
+    return SOMHIP_OK;
+}
+
+int somhip_server_add_model(somhip_server *S, somhip_model *M, int *model_id)
+{
+    if (!S || !M || !model_id) {
+        somhip_set_error("null argument");
+        return SOMHIP_ERR_ARG;
+    }
+    if (M->ctx != S->ctx) {
+        somhip_set_error("model belongs to another context");
+        return SOMHIP_ERR_ARG;
+    }
+    if (M->dpad > 32) {
+        somhip_set_error("serving supports dim <= 32");
+        return SOMHIP_ERR_UNSUPPORTED;
+    }
+    std::lock_guard<std::mutex> g(S->mu);
+    if ((int)S->models.size() >= MAX_MODELS) {
+        somhip_set_error("too many models");
+        return SOMHIP_ERR_ARG;
+    }
+    if (!S->all.empty()) {
+        somhip_set_error("add models before submitting requests");
+        return SOMHIP_ERR_STATE;
+    }
+    int rc = somhip_prepare(M);
+    if (rc) return rc;
+    SOM_HIP_TRY(hipStreamSynchronize(M->ctx->stream));
+    BmuModelDev d = somhip_model_dev(M);
+    const int id = (int)S->models.size();
+    SOM_HIP_TRY(hipMemcpy(S->models_dev + id, &d, sizeof(d), hipMemcpyHostToDevice));
+    S->models.push_back(M);
+    S->model_dpad.push_back(M->dpad);
+    *model_id = id;
+    return SOMHIP_OK;
+}
+
+int somhip_server_submit(somhip_server *S, int model_id, int kind, const float *x_host, const double *rho_host,
+                         int64_t n, void *out_host, int64_t *ticket)
+{
+    if (!S || !ticket || !out_host || n < 0 || (n > 0 && !x_host)) {
+        somhip_set_error("invalid submit arguments");
+        return SOMHIP_ERR_ARG;
+    }
+    if (kind != SOMHIP_SERVE_WINNERS && kind != SOMHIP_SERVE_ACTIVATION) {
+        somhip_set_error("invalid request kind");
+        return SOMHIP_ERR_ARG;
+    }
+    auto R = std::make_shared<Request>();
+    R->t_submit = Clock::now();
+    {
+        std::lock_guard<std::mutex> g(S->mu);
+        if (!S->error.empty()) {
+            somhip_set_error("server failed: " + S->error);
+            return SOMHIP_ERR_HIP;
+        }
+        if (model_id < 0 || model_id >= (int)S->models.size()) {
+            somhip_set_error("unknown model id");
+            return SOMHIP_ERR_ARG;
+        }
+    }
+    somhip_model *M = S->models[model_id];
+    R->model = model_id;
+    R->kind = kind;
+    R->x_host = x_host;
+    R->rho_host = rho_host;
+    R->n = n;
+    R->out_host = out_host;
+    if (kind == SOMHIP_SERVE_ACTIVATION)
+        R->nwg = somhip_scatter_nwg(std::max<int64_t>(n, 1), M->mn, 1, S->ctx->num_cus, ACT_TABLE_BUDGET);
+    size_t b = align_up(sizeof(float) * n * M->dim) + align_up(sizeof(int32_t) * std::max<int64_t>(n, 1));
+    if (rho_host) b += align_up(sizeof(double) * n);
+    if (kind == SOMHIP_SERVE_ACTIVATION)
+        b += align_up(sizeof(double) * M->mn) + 2 * align_up(512) +
+             align_up(sizeof(unsigned long long) * (size_t)R->nwg * M->mn);
+    R->bytes = b;
+    if (b > S->slab.capacity) {
+        somhip_set_error("request exceeds the serving memory budget");
+        return SOMHIP_ERR_ARG;
+    }
+    SOM_HIP_TRY(hipEventCreateWithFlags(&R->ev_up, hipEventDisableTiming));
+    SOM_HIP_TRY(hipEventCreateWithFlags(&R->ev_done, hipEventDisableTiming));
+    {
+        std::lock_guard<std::mutex> g(S->mu);
+        R->ticket = S->next_ticket++;
+        S->pending.push_back(R);
+        S->all[R->ticket] = R;
+        *ticket = R->ticket;
+    }
+    S->cv_work.notify_one();
+    return SOMHIP_OK;
+}
+
+int somhip_server_wait(somhip_server *S, int64_t ticket, double timeout_s, int *done, double *latency_ms)
+{
+    if (!S || !done) return SOMHIP_ERR_ARG;
+    std::unique_lock<std::mutex> g(S->mu);
+    auto it = S->all.find(ticket);
+    if (it == S->all.end()) {
+        somhip_set_error("unknown ticket");
+        return SOMHIP_ERR_ARG;
+    }
+    auto R = it->second;
+    auto pred = [&] { return R->state == DONE || R->state == FAILED; };
+    if (timeout_s > 0) S->cv_done.wait_for(g, std::chrono::duration<double>(timeout_s), pred);
+    if (R->state == FAILED) {
+        somhip_set_error("server failed: " + S->error);
+        return SOMHIP_ERR_HIP;
+    }
+    *done = R->state == DONE ? 1 : 0;
+    if (*done) {
+        if (latency_ms)
+            *latency_ms = std::chrono::duration<double, std::milli>(R->t_done - R->t_submit).count();
+        if (R->ev_up) (void)hipEventDestroy(R->ev_up);
+        if (R->ev_done) (void)hipEventDestroy(R->ev_done);
+        R->ev_up = R->ev_done = nullptr;
+        S->all.erase(it);
+    }
+    return SOMHIP_OK;
+}
+
+int somhip_server_stats(somhip_server *S, int64_t *completed, int64_t *launches, int64_t *tiles, int64_t *in_flight,
+                        size_t *memory_in_use)
+{
+    if (!S) return SOMHIP_ERR_ARG;
+    std::lock_guard<std::mutex> g(S->mu);
+    if (completed) *completed = S->completed.load();
+    if (launches) *launches = S->launches.load();
+    if (tiles) *tiles = S->tiles_launched.load();
+    if (in_flight) *in_flight = (int64_t)S->pending.size() + (int64_t)S->active.size();
+    if (memory_in_use) *memory_in_use = S->slab.used;
+    return SOMHIP_OK;
+}
+
+}  // extern "C"

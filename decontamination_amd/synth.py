@@ -98,3 +98,29 @@ def mixture_dataset(n=100_000, dim=8, k=5, sigma=0.5, seed=0):
 
 def uniform_weights(mn, dim, low=-1.0, high=1.0, seed=0):
     return np.random.default_rng(seed).uniform(low, high, (mn, dim)).astype(np.float32)
+
+
+def catalogue_rows(n, n_lognormal, n_gauss, seed=0, block=1 << 22):
+    """(n, n_lognormal + n_gauss + 2) FP32 catalogue rows from the same feature distributions as
+    systematics_dataset, at uniformly random sky positions (cos(theta) uniform in [-1, 1]), each column
+    standardised with the moments of the distribution it is drawn from."""
+    rng = np.random.default_rng(seed)
+    dim = n_lognormal + n_gauss + 2
+    out = np.empty((n, dim), dtype=np.float32)
+    # analytic moments: lognormal(0, s): mean exp(s^2/2), var (exp(s^2)-1) exp(s^2); sin(theta) with
+    # z uniform: mean pi/4, E[sin^2] = 2/3; z: mean 0, var 1/3
+    s2 = 0.25
+    ln_mu = np.exp(s2 / 2.0)
+    ln_sd = np.sqrt((np.exp(s2) - 1.0) * np.exp(s2))
+    sin_mu = np.pi / 4.0
+    sin_sd = np.sqrt(2.0 / 3.0 - sin_mu ** 2)
+    z_sd = np.sqrt(1.0 / 3.0)
+    for b0 in range(0, n, block):
+        b1 = min(n, b0 + block)
+        k = b1 - b0
+        out[b0:b1, :n_lognormal] = (rng.lognormal(0.0, 0.5, (k, n_lognormal)) - ln_mu) / ln_sd
+        out[b0:b1, n_lognormal:n_lognormal + n_gauss] = rng.standard_normal((k, n_gauss))
+        z = rng.uniform(-1.0, 1.0, k)
+        out[b0:b1, -2] = (np.sqrt(1.0 - z * z) - sin_mu) / sin_sd
+        out[b0:b1, -1] = z / z_sd
+    return out

@@ -161,6 +161,38 @@ int somhip_batch_iter_accumulate(somhip_accum *acc, const float *x, const double
 /* Vector-count-mode division (som_batch.py:481-486). */
 int somhip_batch_iter_update(somhip_accum *acc, float *history_out);
 
+/* ------------------------------------------------------------------------------------------------
+ * Serving. No counterpart in the reference (it has no serving loop); built on the same kernels as
+ * somhip_find_bmus / somhip_count_bmus with PREC_BF16_RERANK.
+ *
+ * A server keeps models resident, admits requests while their working memory fits in
+ * `memory_budget` bytes of device memory (one slab allocated at creation), uploads their rows on a
+ * copy stream, and batches the 256-row tiles of every uploaded request into shared launches
+ * (hipGraphs captured per dimension class and tile-count bucket). Results are bit-identical to the
+ * single-request entry points, alone or in any mix. A scheduler thread owns the server's streams.
+ * Host buffers passed to submit must stay valid until the request completes; pinned buffers from
+ * somhip_host_alloc avoid staging copies.
+ * ------------------------------------------------------------------------------------------------ */
+#define SOMHIP_SERVE_WINNERS 0    /* out_host: int32[n] BMU indices (default 0) */
+#define SOMHIP_SERVE_ACTIVATION 1 /* out_host: double[mn] activation map (rho == NULL means ones) */
+
+typedef struct somhip_server somhip_server;
+
+int somhip_host_alloc(size_t bytes, void **ptr);
+int somhip_host_free(void *ptr);
+int somhip_server_create(somhip_ctx *ctx, size_t memory_budget, int arithmetic, somhip_server **out);
+/* Waits for every submitted request, then frees the server. */
+int somhip_server_destroy(somhip_server *server);
+/* Register a model (its weights are prepared once; models must be added before the first submit). */
+int somhip_server_add_model(somhip_server *server, somhip_model *model, int *model_id);
+int somhip_server_submit(somhip_server *server, int model_id, int kind, const float *x_host, const double *rho_host,
+                         int64_t n, void *out_host, int64_t *ticket);
+/* Waits up to timeout_s (0 = poll) for a ticket; on completion sets *done = 1 and the latency from
+ * submit to the result being in out_host, and forgets the ticket. */
+int somhip_server_wait(somhip_server *server, int64_t ticket, double timeout_s, int *done, double *latency_ms);
+int somhip_server_stats(somhip_server *server, int64_t *completed, int64_t *launches, int64_t *tiles,
+                        int64_t *in_flight, size_t *memory_in_use);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,88 @@
+"""Serving: each result is the same alone and in the mix (bit-identical), and equals the
+single-request entry points; admission under a small memory budget still completes every request."""
+import numpy as np
+import pytest
+
+import decontamination_amd as dam
+from decontamination_amd.serving import SOMServer, pinned_empty
+
+pytestmark = pytest.mark.gpu
+
+
+def make_soms():
+    rng = np.random.default_rng(7)
+    soms = []
+    for (m, n, d, topo) in [(16, 16, 16, "hexagonal"), (10, 12, 32, "square"), (32, 32, 16, "hexagonal")]:
+        s = dam.SOM_Batch(m, n, d, topology=topo, precision="bf16_rerank")
+        s._weights[...] = rng.standard_normal((m * n, d)).astype(np.float32)
+        soms.append(s)
+    return soms
+
+
+def make_requests(soms):
+    rng = np.random.default_rng(11)
+    sizes = [0, 1, 255, 256, 257, 1000, 4096, 50_000, 200_001]
+    reqs = []
+    for i, rows in enumerate(sizes * 2):
+        mi = i % len(soms)
+        d = soms[mi].dim
+        x = pinned_empty((rows, d), np.float32)
+        x[...] = rng.standard_normal((rows, d)).astype(np.float32)
+        if rows > 10:
+            x[3] = np.nan
+        kind = "winners" if i % 2 == 0 else "activation"
+        rho = None
+        if kind == "activation" and i % 4 == 1 and rows > 0:
+            rho = rng.random(rows) * 2.0
+            rho[::9] = 0.0
+        reqs.append((mi, kind, x, rho))
+    return reqs
+
+
+def run_alone(server, reqs):
+    out = []
+    for mi, kind, x, rho in reqs:
+        t = server.submit(mi, kind, x, rho)
+        out.append(server.wait(t)[0].copy())
+    return out
+
+
+def run_mixed(server, reqs):
+    tickets = [server.submit(mi, kind, x, rho) for mi, kind, x, rho in reqs]
+    return [server.wait(t)[0].copy() for t in tickets]
+
+
+def test_alone_equals_mixed_equals_single_entry_points():
+    soms = make_soms()
+    reqs = make_requests(soms)
+    with SOMServer(soms, memory_budget=2 << 30) as server:
+        a = run_alone(server, reqs)
+        b = run_mixed(server, reqs)
+        st = server.stats()
+    assert st["completed"] == 2 * len(reqs)
+    for (mi, kind, x, rho), ra, rb in zip(reqs, a, b):
+        assert np.array_equal(ra, rb, equal_nan=True)
+        som = soms[mi]
+        if kind == "winners":
+            assert ra.dtype == np.int32 and ra.shape == (x.shape[0],)
+            if x.shape[0]:
+                assert np.array_equal(ra, som.get_winners(np.asarray(x)))
+        else:
+            assert ra.shape == (som.m, som.n) and ra.dtype == np.float32
+            assert np.array_equal(ra, som.get_activation_map(np.asarray(x), rho))
+
+
+def test_small_budget_admission():
+    soms = make_soms()[:1]
+    rng = np.random.default_rng(3)
+    xs = [np.ascontiguousarray(rng.standard_normal((30_000, 16)).astype(np.float32)) for _ in range(12)]
+    ref = [soms[0].get_winners(x) for x in xs]
+    # one request needs about 30000*(64+4) bytes = 2 MB: a 3 MB budget admits one at a time
+    with SOMServer(soms, memory_budget=3 << 20) as server:
+        tickets = [server.submit(0, "winners", x) for x in xs]
+        res = [server.wait(t) for t in tickets]
+        with pytest.raises(ValueError):
+            server.submit(0, "winners", np.zeros((100_000, 16), np.float32))
+    for (r, lat), e in zip(res, ref):
+        assert np.array_equal(r, e)
+        assert lat > 0.0
