@@ -47,6 +47,8 @@ def parse():
     ap.add_argument("--serve-seconds", type=float, default=20.0)
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--serve-budget-gb", type=float, default=64.0)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run one device-to-device copy of the dataset (reference kernel for HBM counters)")
     return ap.parse_args()
 
 
@@ -279,6 +281,14 @@ def main():
     ctx.synchronize()
     h2d_s = time.perf_counter() - t
     log(f"uploaded {X.nbytes / 1e9:.2f} GB in {h2d_s:.2f}s")
+
+    if args.calibrate_copy:
+        tmp = DeviceArray.empty(ctx, X.shape, np.float32)
+        from decontamination_amd import _lib
+        _lib.call("somhip_memcpy_d2d", ctx.handle, _lib.vp(tmp.ptr), _lib.vp(x_dev.ptr), X.nbytes)
+        ctx.synchronize()
+        tmp.free()
+        log(f"calibration copy of {X.nbytes} bytes done")
 
     results = {}
     for prec in args.precisions.split(","):

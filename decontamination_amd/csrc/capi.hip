@@ -12,6 +12,8 @@ int somhip_launch_absmax(hipStream_t st, const float *x, int dim, int dpad, cons
                          unsigned int *bounds, int num_cus);
 int somhip_launch_shifts(hipStream_t st, const unsigned int *bounds, int dim, int has_x, int has_rho, int64_t n,
                          int *shifts);
+int somhip_launch_shifts_parts(hipStream_t st, const float *parts, int64_t rows, int dpad, int dim,
+                               const unsigned int *rho_bound, int64_t n, int *shifts);
 int somhip_scatter_nwg(int64_t n, int mn, int nv, int num_cus, size_t table_budget);
 int somhip_launch_scatter(hipStream_t st, const int32_t *bmu, const float *x, int dim, const double *rho, int rho_f32,
                           int64_t n, const int *shifts, int mn, int nv, int nwg, unsigned long long *partial,
@@ -352,6 +354,7 @@ int somhip_model_create(somhip_ctx *C, int m, int n, int dim, int topology, cons
     SOM_ALLOC(M->norms_nat, sizeof(float) * slots);
     SOM_ALLOC(M->frag_bf, sizeof(unsigned short) * slots * M->dpad * 2);
     SOM_ALLOC(M->frag_f32, sizeof(float) * slots * M->dpad);
+    SOM_ALLOC(M->wperm, sizeof(float) * slots * M->dpad);
     SOM_ALLOC(M->perm, sizeof(int) * slots);
 #undef SOM_ALLOC
     std::vector<int> perm = make_perm(m, n, M->nblk);
@@ -370,7 +373,8 @@ int somhip_model_destroy(somhip_model *M)
 {
     if (!M) return SOMHIP_OK;
     if (M->ctx) (void)hipStreamSynchronize(M->ctx->stream);
-    void *ptrs[] = {M->w, M->topo, M->centre, M->wmax, M->scal, M->norms_perm, M->norms_nat, M->frag_bf, M->frag_f32, M->perm};
+    void *ptrs[] = {M->w,       M->topo,      M->centre,  M->wmax,     M->scal, M->norms_perm,
+                    M->norms_nat, M->frag_bf, M->frag_f32, M->wperm, M->perm};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     delete M;
@@ -420,7 +424,7 @@ int somhip_find_bmus(somhip_model *M, const float *x, int64_t n, int precision, 
         somhip_set_error("invalid x/out/n");
         return SOMHIP_ERR_ARG;
     }
-    return somhip_launch_bmu(M, x, n, precision, arithmetic, 0, 0, out, nullptr, nullptr);
+    return somhip_launch_bmu(M, x, n, precision, arithmetic, 0, 0, out, nullptr, nullptr, nullptr);
 }
 
 int somhip_count_bmus(somhip_model *M, const float *x, const double *rho, int64_t n, int precision, int arithmetic,
@@ -441,7 +445,7 @@ int somhip_count_bmus(somhip_model *M, const float *x, const double *rho, int64_
         (rc = W.table.ensure((size_t)nwg * M->mn * 8)))
         return rc;
     int32_t *c1 = (int32_t *)W.c1.p;
-    if ((rc = somhip_launch_bmu(M, x, n, precision, arithmetic, 0, 0, c1, nullptr, nullptr))) return rc;
+    if ((rc = somhip_launch_bmu(M, x, n, precision, arithmetic, 0, 0, c1, nullptr, nullptr, nullptr))) return rc;
     {
         TimerScope ts(C, TK_ABSMAX);
         if ((rc = somhip_launch_absmax(C->stream, nullptr, 0, 16, rho, 0, rho ? n : 0, (unsigned int *)W.bounds.p,
@@ -475,7 +479,7 @@ int somhip_compute_errors(somhip_model *M, const float *x, const double *rho, in
         return rc;
     int32_t *c1 = (int32_t *)W.c1.p, *c2 = (int32_t *)W.c2.p;
     double *d1 = (double *)W.d1.p;
-    if ((rc = somhip_launch_bmu(M, x, n, precision, arithmetic, 1, -1, c1, c2, d1))) return rc;
+    if ((rc = somhip_launch_bmu(M, x, n, precision, arithmetic, 1, -1, c1, c2, d1, nullptr))) return rc;
     const double penalty = M->topology == SOMHIP_TOPOLOGY_SQUARE ? 2.0 : 1.0;
     TimerScope ts(C, TK_ERRORS);
     return somhip_launch_errors(C->stream, c1, c2, d1, rho, n, M->topo, penalty, (double *)W.part.p, nparts, errors);
@@ -540,16 +544,19 @@ int somhip_batch_epoch_accumulate(somhip_accum *A, const float *x, const double 
     const int nwg = somhip_scatter_nwg(n, M->mn, nv, C->num_cus, SCATTER_TABLE_BUDGET);
     const int nparts = somhip_errors_nparts(n);
     Workspace &W = A->ws;
+    const int64_t xrows = somhip_bmu_xmax_rows(precision, M->dpad, n);
     if ((rc = W.c1.ensure(sizeof(int32_t) * n)) || (rc = W.c2.ensure(sizeof(int32_t) * n)) ||
         (rc = W.d1.ensure(sizeof(double) * n)) || (rc = W.part.ensure(sizeof(double) * 3 * nparts)) ||
         (rc = W.bounds.ensure(512)) || (rc = W.shifts.ensure(512)) ||
-        (rc = W.table.ensure((size_t)nwg * M->mn * nv * 8)))
+        (rc = W.table.ensure((size_t)nwg * M->mn * nv * 8)) ||
+        (rc = W.xmax.ensure(sizeof(float) * (size_t)xrows * M->dpad)))
         return rc;
     int32_t *c1 = (int32_t *)W.c1.p, *c2 = (int32_t *)W.c2.p;
     double *d1 = (double *)W.d1.p;
     const bool need_err = errors != nullptr;
+    // the BMU search also writes per-wave max |x_k| rows, from which the fixed-point scales follow
     if ((rc = somhip_launch_bmu(M, x, n, precision, arithmetic, need_err ? 1 : 0, -1, c1, need_err ? c2 : nullptr,
-                                need_err ? d1 : nullptr)))
+                                need_err ? d1 : nullptr, (float *)W.xmax.p)))
         return rc;
     if (need_err) {
         const double penalty = M->topology == SOMHIP_TOPOLOGY_SQUARE ? 2.0 : 1.0;
@@ -560,11 +567,11 @@ int somhip_batch_epoch_accumulate(somhip_accum *A, const float *x, const double 
     }
     {
         TimerScope ts(C, TK_ABSMAX);
-        if ((rc = somhip_launch_absmax(C->stream, x, M->dim, M->dpad, rho, 1, n, (unsigned int *)W.bounds.p,
-                                       C->num_cus)))
+        if (rho &&
+            (rc = somhip_launch_absmax(C->stream, nullptr, 0, 16, rho, 1, n, (unsigned int *)W.bounds.p, C->num_cus)))
             return rc;
-        if ((rc = somhip_launch_shifts(C->stream, (const unsigned int *)W.bounds.p, M->dim, 1, rho != nullptr, n,
-                                       (int *)W.shifts.p)))
+        if ((rc = somhip_launch_shifts_parts(C->stream, (const float *)W.xmax.p, xrows, M->dpad, M->dim,
+                                             rho ? (const unsigned int *)W.bounds.p : nullptr, n, (int *)W.shifts.p)))
             return rc;
     }
     TimerScope ts(C, TK_SCATTER);
@@ -621,7 +628,7 @@ int somhip_batch_iter_accumulate(somhip_accum *A, const float *x, const double *
         (rc = A->upd.ensure(sizeof(double) * (size_t)nsplit * M->mn * (M->dim + 1))))
         return rc;
     int32_t *c1 = (int32_t *)W.c1.p;
-    if ((rc = somhip_launch_bmu(M, x, n, precision, arithmetic, 0, -1, c1, nullptr, nullptr))) return rc;
+    if ((rc = somhip_launch_bmu(M, x, n, precision, arithmetic, 0, -1, c1, nullptr, nullptr, nullptr))) return rc;
     TimerScope ts(C, TK_ITER);
     return somhip_launch_iter(C->stream, c1, x, M->dim, M->dpad, rho, n, cur_vector, n_vectors, sigma0, M->topo, M->mn,
                               (double *)A->upd.p, nsplit, A->acc);

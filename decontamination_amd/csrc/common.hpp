@@ -105,6 +105,7 @@ struct BmuModelDev {
     const float *centre;    // weight centroid (dpad)
     const float *wmax;      // max_j |w'_jk| (dpad)
     const float *scal;      // [0] max_j |w'_j|^2
+    const float *wperm;     // FP32 centred weights per permuted slot, (slots, dpad) row-major
     int nblk, mn, dim;
 };
 
@@ -120,7 +121,7 @@ struct BmuTile {
     int default_index;
 };
 
-constexpr int SOM_TILE_ROWS = 256;     // rows per BF16 workgroup (4 waves x 2 groups x 32)
+constexpr int SOM_TILE_ROWS = 512;     // rows per BF16 workgroup (8 waves x 2 groups x 32)
 constexpr int FB_TILE_SHIFT = 40;      // serving fallback entry = (tile << 40) | row
 
 // ------------------------------------------------------------------------------------------------
@@ -161,10 +162,11 @@ struct DevBuf {
 
 // Per-call device workspace (BMU outputs, reduction partials, fixed-point scales and tables).
 struct Workspace {
-    DevBuf c1, c2, d1, part, bounds, shifts, table;
+    DevBuf c1, c2, d1, part, bounds, shifts, table, xmax;
     void release()
     {
         c1.release(); c2.release(); d1.release(); part.release(); bounds.release(); shifts.release(); table.release();
+        xmax.release();
     }
 };
 
@@ -215,6 +217,7 @@ struct somhip_model {
     float *norms_nat = nullptr;     // (nblk*32) |w'|^2 in natural order (padding slots = +inf)
     unsigned short *frag_bf = nullptr;  // bf16 A fragments [blk][kc][hi,lo][64 lanes][8]
     float *frag_f32 = nullptr;      // FP32 A fragments [blk][dpad/8][64 lanes][4]
+    float *wperm = nullptr;         // FP32 centred weights per permuted slot [slot][dpad]
     int *perm = nullptr;            // (nblk*32) slot -> neuron (or -1 for padding)
 };
 
@@ -230,8 +233,11 @@ int somhip_timer_collect(somhip_ctx *C);
 
 // Internal launchers (return SOMHIP_OK or an error code).
 int somhip_prepare(somhip_model *mdl);
+// xmax_part (nullable): per-wave rows of dpad floats with max |x_k| over finite values, for the
+// fixed-point scales of training; somhip_bmu_xmax_rows gives the number of rows written.
 int somhip_launch_bmu(somhip_model *mdl, const float *x, int64_t n, int precision, int arithmetic, int need_c2,
-                      int default_index, int32_t *c1, int32_t *c2, double *d1);
+                      int default_index, int32_t *c1, int32_t *c2, double *d1, float *xmax_part);
+int64_t somhip_bmu_xmax_rows(int precision, int dpad, int64_t n);
 BmuModelDev somhip_model_dev(const somhip_model *M);
 // Serving form of the BF16 search: `grid` tiles from `tiles` (n == 0 tiles are skipped), then the exact
 // fallback over the flagged (tile, row) entries with a fixed grid; fb_count must be zero on entry.

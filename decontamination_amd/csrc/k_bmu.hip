@@ -24,7 +24,7 @@ namespace {
 
 constexpr int WAVE = 64;
 constexpr int BMU_WAVES = 4;                 // waves per workgroup of the MFMA kernels
-constexpr int CAPL = 8;                      // candidate slots per lane (16 per sample)
+constexpr int CAPL = 20;                     // recorded candidate neurons per lane (BF16 window)
 constexpr float U32 = 5.9604644775390625e-8f;  // 2^-24
 
 // Each lane needs its partner (lane ^ 32) value: both hold the same sample column.
@@ -41,11 +41,14 @@ __global__ __launch_bounds__(256) void bmu_exact_kernel(const float *__restrict_
                                                         const unsigned int *__restrict__ list_count,
                                                         const float *__restrict__ w, int mn, int default_index,
                                                         int32_t *__restrict__ c1, int32_t *__restrict__ c2,
-                                                        double *__restrict__ d1)
+                                                        double *__restrict__ d1, float *__restrict__ xmax_part)
 {
     constexpr int CH = 128;  // neurons per LDS chunk
     __shared__ float ws[CH * DP];
     const int64_t total = list ? (int64_t)(*list_count) : n;
+    float amax[DP];
+#pragma unroll
+    for (int k = 0; k < DP; k++) amax[k] = 0.f;
     // Grid-stride over samples; every thread of the block takes part in the chunk loads.
     for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < total; base += (int64_t)gridDim.x * blockDim.x) {
         const int64_t t = base + threadIdx.x;
@@ -53,7 +56,11 @@ __global__ __launch_bounds__(256) void bmu_exact_kernel(const float *__restrict_
         const int64_t i = active ? (list ? list[t] : t) : 0;
         float xr[DP];
 #pragma unroll
-        for (int k = 0; k < DP; k++) xr[k] = (active && k < dim) ? x[i * dim + k] : 0.f;
+        for (int k = 0; k < DP; k++) {
+            xr[k] = (active && k < dim) ? x[i * dim + k] : 0.f;
+            const float a = fabsf(xr[k]);
+            amax[k] = fmaxf(amax[k], a < __builtin_inff() ? a : 0.f);
+        }
         double md1 = 1.0e99, md2 = 1.0e99;
         int mi1 = -1, mi2 = -1;
         for (int j0 = 0; j0 < mn; j0 += CH) {
@@ -85,6 +92,21 @@ __global__ __launch_bounds__(256) void bmu_exact_kernel(const float *__restrict_
             if (d1) d1[i] = mi1 >= 0 ? md1 : __builtin_nan("");
         }
     }
+    if (xmax_part) {
+        const int lane = threadIdx.x & (WAVE - 1);
+#pragma unroll
+        for (int k = 0; k < DP; k++) {
+            float v = amax[k];
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, WAVE));
+            amax[k] = v;
+        }
+        if (lane == 0) {
+            const int64_t wid = (int64_t)blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
+#pragma unroll
+            for (int k = 0; k < DP; k++) xmax_part[wid * DP + k] = amax[k];
+        }
+    }
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -94,6 +116,7 @@ __global__ __launch_bounds__(256) void bmu_exact_kernel(const float *__restrict_
 template <int KC>
 struct SampleLoad {
     float xc[KC][8];  // centred FP32 values of dims kc*16 + 8*(lane/32) + e
+    float ax[KC][8];  // |x| of the same dims (0 when not finite), for the fixed-point scales
 };
 
 template <int KC>
@@ -108,7 +131,32 @@ __device__ __forceinline__ void load_sample(const float *__restrict__ x, int64_t
             const int k = kc * 16 + hb + e;
             float v = (s < n && k < dim) ? x[s * dim + k] : 0.f;
             L.xc[kc][e] = (k < dim) ? __fsub_rn(v, centre[k]) : 0.f;
+            const float a = fabsf(v);
+            L.ax[kc][e] = a < __builtin_inff() ? a : 0.f;
         }
+    }
+}
+
+// Per-wave max of |x| per dimension: lanes l and l^32 hold different dims, the 32 lanes of a half
+// hold the same dims for different samples. Writes DP floats for this wave (plain stores).
+template <int KC>
+__device__ __forceinline__ void wave_dim_max(float (&amax)[KC][8], int lane, float *__restrict__ part)
+{
+#pragma unroll
+    for (int kc = 0; kc < KC; kc++)
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+            float v = amax[kc][e];
+#pragma unroll
+            for (int o = 16; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, WAVE));
+            amax[kc][e] = v;
+        }
+    if ((lane & 31) == 0) {
+        const int64_t wid = (int64_t)blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
+#pragma unroll
+        for (int kc = 0; kc < KC; kc++)
+#pragma unroll
+            for (int e = 0; e < 8; e++) part[wid * (KC * 16) + kc * 16 + 8 * (lane >> 5) + e] = amax[kc][e];
     }
 }
 
@@ -119,7 +167,7 @@ template <int DP, int G>
 __global__ __launch_bounds__(BMU_WAVES * WAVE) void bmu_fp32_kernel(
     const float *__restrict__ x, int64_t n, int dim, const float *__restrict__ centre,
     const f32x4 *__restrict__ frag, const f32x4 *__restrict__ norms4, int nblk, int mn, int default_index,
-    int32_t *__restrict__ c1, int32_t *__restrict__ c2, double *__restrict__ d1)
+    int32_t *__restrict__ c1, int32_t *__restrict__ c2, double *__restrict__ d1, float *__restrict__ xmax_part)
 {
     constexpr int KS = DP / 2;  // MFMA k-steps of 2
     const int lane = threadIdx.x & (WAVE - 1);
@@ -131,6 +179,9 @@ __global__ __launch_bounds__(BMU_WAVES * WAVE) void bmu_fp32_kernel(
     // B operand: b[g][s] = -2 * x'[sample][2s + khalf]
     float b[G][KS];
     float xn[G];  // |x'|^2 partial (this lane's dims), combined below
+    float amax[KS];
+#pragma unroll
+    for (int st = 0; st < KS; st++) amax[st] = 0.f;
 #pragma unroll
     for (int g = 0; g < G; g++) {
         const int64_t s = sbase + g * 32 + col;
@@ -138,11 +189,28 @@ __global__ __launch_bounds__(BMU_WAVES * WAVE) void bmu_fp32_kernel(
 #pragma unroll
         for (int st = 0; st < KS; st++) {
             const int k = 2 * st + khalf;
-            float v = (s < n && k < dim) ? __fsub_rn(x[s * dim + k], centre[k]) : 0.f;
+            const float raw = (s < n && k < dim) ? x[s * dim + k] : 0.f;
+            const float a = fabsf(raw);
+            amax[st] = fmaxf(amax[st], a < __builtin_inff() ? a : 0.f);
+            float v = (k < dim) ? __fsub_rn(raw, centre[k]) : 0.f;
             b[g][st] = -2.f * v;
             acc = fmaf(v, v, acc);
         }
         xn[g] = acc + xchg32(acc);
+    }
+    if (xmax_part) {
+#pragma unroll
+        for (int st = 0; st < KS; st++) {
+            float v = amax[st];
+#pragma unroll
+            for (int o = 16; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, WAVE));
+            amax[st] = v;
+        }
+        if ((lane & 31) == 0) {
+            const int64_t wid = (int64_t)blockIdx.x * BMU_WAVES + wave;
+#pragma unroll
+            for (int st = 0; st < KS; st++) xmax_part[wid * DP + 2 * st + khalf] = amax[st];
+        }
     }
 
     float m1[G], m2[G];
@@ -227,36 +295,75 @@ __global__ __launch_bounds__(BMU_WAVES * WAVE) void bmu_fp32_kernel(
 
 // ------------------------------------------------------------------------------------------------
 // BF16 3-term split with candidate window and exact rerank.
+//
+// A workgroup of NW waves (NW*G*32 samples, one tile) streams the 32-neuron blocks through LDS in
+// chunks of CB blocks (A fragments + norms), double-buffered: while the waves compute chunk c from one
+// buffer, every thread holds its share of chunk c+1 in registers and writes it to the other buffer
+// after its last block of chunk c; one barrier per chunk.
 // ------------------------------------------------------------------------------------------------
+constexpr int NW_BF16 = 8;      // waves per workgroup
+constexpr int G_BF16 = 2;       // 32-sample groups per wave
+constexpr int CHUNK_KB = 8;             // (32-neuron block) x (16-wide K slice) units per chunk
+constexpr int FRAG_UNITS = CHUNK_KB * 2 * 64;  // 16-byte units of A fragments per chunk
+static_assert(NW_BF16 * G_BF16 * 32 == SOM_TILE_ROWS, "tile rows must match the BF16 workgroup");
+
+template <int KC>
+struct Bf16Smem {
+    static constexpr int CB = CHUNK_KB / KC;  // blocks per chunk
+    uint4 frag[2][FRAG_UNITS];          // [buf][cb][kc][hi,lo][lane] 16 B
+    float norms[2][CB * 32];            // [buf][cb][row]
+    unsigned short cand[NW_BF16][G_BF16][CAPL][WAVE];
+};
+
 template <int DP, int G, int MODE, bool NEED_C2>
 __device__ __forceinline__ void bmu_bf16_body(const BmuModelDev &Md, const float *__restrict__ x, int64_t n,
-                                              int64_t sbase, int default_index, int32_t *__restrict__ c1,
+                                              int64_t tile_row0, int default_index, int32_t *__restrict__ c1,
                                               int32_t *__restrict__ c2, double *__restrict__ d1,
                                               int64_t *__restrict__ fb_list, unsigned int *__restrict__ fb_count,
-                                              int64_t fb_tag, unsigned short (*cw)[CAPL][WAVE])
+                                              int64_t fb_tag, Bf16Smem<DP / 16> &sm, float *__restrict__ xmax_part)
 {
     constexpr int KC = DP / 16;
+    constexpr int CB = Bf16Smem<KC>::CB;
+    constexpr int NORM_UNITS = CB * 8;   // 16-byte units of norms per chunk
+    constexpr int NT = NW_BF16 * WAVE;   // threads per workgroup
+    constexpr int FU = FRAG_UNITS / NT;  // fragment units per thread
+    static_assert(FRAG_UNITS % NT == 0 && NORM_UNITS <= NT, "chunk layout");
     const float *__restrict__ centre = Md.centre;
     const float *__restrict__ wmax = Md.wmax;
     const float *__restrict__ scal = Md.scal;
-    const bf16x8 *__restrict__ frag = Md.frag;
-    const f32x4 *__restrict__ norms4 = Md.norms4;
+    const uint4 *__restrict__ gfrag = (const uint4 *)Md.frag;
+    const uint4 *__restrict__ gnorm = (const uint4 *)Md.norms4;
     const int *__restrict__ perm = Md.perm;
     const float *__restrict__ w = Md.w;
-    const int nblk = Md.nblk, mn = Md.mn, dim = Md.dim;
-    (void)mn;
-    const int lane = threadIdx.x & (WAVE - 1);
+    const int nblk = Md.nblk, dim = Md.dim;
+    const int tid = threadIdx.x;
+    const int lane = tid & (WAVE - 1);
+    const int wave = tid / WAVE;
     const int col = lane & 31;
     const int khalf = lane >> 5;
+    const int64_t sbase = tile_row0 + (int64_t)wave * (G * 32);
+    unsigned short(*cw)[CAPL][WAVE] = sm.cand[wave];
+    const int nchunks = (nblk + CB - 1) / CB;
+    const int64_t frag_units_total = (int64_t)nblk * KC * 2 * 64;
+    const int64_t norm_units_total = (int64_t)nblk * 8;
 
     const float nwmax = scal[0];
     bf16x8 bh[G][KC], bl[G][KC];
     float E[G], K[G];
+    float amax[KC][8];
+#pragma unroll
+    for (int kc = 0; kc < KC; kc++)
+#pragma unroll
+        for (int e = 0; e < 8; e++) amax[kc][e] = 0.f;
 #pragma unroll
     for (int g = 0; g < G; g++) {
         const int64_t s = sbase + g * 32 + col;
         SampleLoad<KC> L;
         load_sample<KC>(x, n, dim, s, lane, centre, L);
+#pragma unroll
+        for (int kc = 0; kc < KC; kc++)
+#pragma unroll
+            for (int e = 0; e < 8; e++) amax[kc][e] = fmaxf(amax[kc][e], L.ax[kc][e]);
         float sx = 0.f, kk = 0.f;
 #pragma unroll
         for (int kc = 0; kc < KC; kc++) {
@@ -275,22 +382,48 @@ __device__ __forceinline__ void bmu_bf16_body(const BmuModelDev &Md, const float
         }
         sx += xchg32(sx);
         kk += xchg32(kk);
-        // Error bound of a_j = |w'_j|^2 - 2 x'.w'_j against its exact value (see the design notes):
-        // split truncation 6.04*2^-16*Sx, MFMA FP32 accumulation (3*DP+6)*2^-24*(nwmax + 2.05*Sx),
-        // centring 4*2^-24*(K + nwmax); x1.25 safety; rounding of sx itself covered by the factor.
+        // Error bound of a_j = |w'_j|^2 - 2 x'.w'_j against its exact value (derivation in the file
+        // header of the design notes): split truncation 6.04*2^-16*Sx, MFMA FP32 accumulation
+        // (3*DP+6)*2^-24*(nwmax + 2.05*Sx), centring 4*2^-24*(K + nwmax); x1.25 safety.
         const float split = 6.04f * 1.52587890625e-5f * sx;
         const float accum = (3.f * DP + 6.f) * U32 * (nwmax + 2.05f * sx);
         const float cent = 4.f * U32 * (kk + nwmax);
         E[g] = 1.25f * (split + accum + cent) + 1e-30f;
         K[g] = kk * (1.f + (DP + 2) * U32);
     }
+    if (xmax_part) wave_dim_max<KC>(amax, lane, xmax_part);
 
-    // ---------------- pass 1: top-2 of per-lane block minima ----------------
-    float m1[G], m2[G];
+    // chunk streaming ---------------------------------------------------------------------------
+    uint4 pf[FU];
+    uint4 pn = make_uint4(0u, 0u, 0u, 0u);
+    auto load_chunk = [&](int c) {
+        const int64_t fb = (int64_t)c * FRAG_UNITS;
 #pragma unroll
-    for (int g = 0; g < G; g++) m1[g] = m2[g] = __builtin_inff();
-
-    auto block_acc = [&](int blk, int g, const bf16x8 (&ah)[KC], const bf16x8 (&al)[KC], const f32x16 &cin) {
+        for (int u = 0; u < FU; u++) {
+            const int64_t idx = fb + tid + (int64_t)u * NT;
+            pf[u] = idx < frag_units_total ? gfrag[idx] : make_uint4(0u, 0u, 0u, 0u);
+        }
+        const int64_t nidx = (int64_t)c * NORM_UNITS + tid;
+        if (tid < NORM_UNITS && nidx < norm_units_total) pn = gnorm[nidx];
+    };
+    auto store_chunk = [&](int buf) {
+#pragma unroll
+        for (int u = 0; u < FU; u++) sm.frag[buf][tid + u * NT] = pf[u];
+        if (tid < NORM_UNITS) ((uint4 *)sm.norms[buf])[tid] = pn;
+    };
+    auto read_block = [&](int buf, int cb, bf16x8 (&ah)[KC], bf16x8 (&al)[KC], f32x16 &cin) {
+#pragma unroll
+        for (int kc = 0; kc < KC; kc++) {
+            ah[kc] = __builtin_bit_cast(bf16x8, sm.frag[buf][((cb * KC + kc) * 2 + 0) * 64 + lane]);
+            al[kc] = __builtin_bit_cast(bf16x8, sm.frag[buf][((cb * KC + kc) * 2 + 1) * 64 + lane]);
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const f32x4 t = *(const f32x4 *)&sm.norms[buf][cb * 32 + 8 * q + 4 * khalf];
+            cin[4 * q + 0] = t[0]; cin[4 * q + 1] = t[1]; cin[4 * q + 2] = t[2]; cin[4 * q + 3] = t[3];
+        }
+    };
+    auto block_acc = [&](int g, const bf16x8 (&ah)[KC], const bf16x8 (&al)[KC], const f32x16 &cin) {
         f32x16 acc = cin;
 #pragma unroll
         for (int kc = 0; kc < KC; kc++) {
@@ -300,41 +433,46 @@ __device__ __forceinline__ void bmu_bf16_body(const BmuModelDev &Md, const float
         }
         return acc;
     };
-    auto load_block = [&](int blk, bf16x8 (&ah)[KC], bf16x8 (&al)[KC], f32x16 &cin) {
-#pragma unroll
-        for (int kc = 0; kc < KC; kc++) {
-            ah[kc] = frag[(((int64_t)blk * KC + kc) * 2 + 0) * WAVE + lane];
-            al[kc] = frag[(((int64_t)blk * KC + kc) * 2 + 1) * WAVE + lane];
+    auto stream = [&](auto &&per_block) {
+        __syncthreads();  // the buffers may still be read by the previous pass
+        if (nchunks > 0) {
+            load_chunk(0);
+            store_chunk(0);
         }
+        __syncthreads();
+        for (int c = 0; c < nchunks; c++) {
+            const int buf = c & 1;
+            if (c + 1 < nchunks) load_chunk(c + 1);
+            const int b0 = c * CB;
+            const int b1 = min(nblk, b0 + CB);
+            for (int blk = b0; blk < b1; blk++) {
+                bf16x8 ah[KC], al[KC];
+                f32x16 cin;
+                read_block(buf, blk - b0, ah, al, cin);
+                f32x16 acc[G];
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
-            f32x4 t = norms4[blk * 8 + 2 * q + khalf];
-            cin[4 * q + 0] = t[0]; cin[4 * q + 1] = t[1]; cin[4 * q + 2] = t[2]; cin[4 * q + 3] = t[3];
+                for (int g = 0; g < G; g++) acc[g] = block_acc(g, ah, al, cin);
+                per_block(blk, acc);
+            }
+            if (c + 1 < nchunks) store_chunk(buf ^ 1);
+            __syncthreads();
         }
     };
 
-    bf16x8 ah[KC], al[KC];
-    f32x16 cin;
-    if (nblk > 0) load_block(0, ah, al, cin);
-    for (int blk = 0; blk < nblk; blk++) {
-        bf16x8 nah[KC], nal[KC];
-        f32x16 ncin;
-        if (blk + 1 < nblk) load_block(blk + 1, nah, nal, ncin);
+    // ---------------- pass 1: top-2 of per-lane block minima ----------------
+    float m1[G], m2[G];
+#pragma unroll
+    for (int g = 0; g < G; g++) m1[g] = m2[g] = __builtin_inff();
+    stream([&](int blk, f32x16 (&acc)[G]) {
 #pragma unroll
         for (int g = 0; g < G; g++) {
-            f32x16 acc = block_acc(blk, g, ah, al, cin);
             float t = __builtin_inff();
 #pragma unroll
-            for (int r = 0; r < 16; r += 2) t = fminf(fminf(t, acc[r]), acc[r + 1]);
+            for (int r = 0; r < 16; r += 2) t = fminf(fminf(t, acc[g][r]), acc[g][r + 1]);
             m2[g] = __builtin_amdgcn_fmed3f(m1[g], m2[g], t);
             m1[g] = fminf(m1[g], t);
         }
-        if (blk + 1 < nblk) {
-#pragma unroll
-            for (int kc = 0; kc < KC; kc++) { ah[kc] = nah[kc]; al[kc] = nal[kc]; }
-            cin = ncin;
-        }
-    }
+    });
 
     // Threshold: every neuron whose exact distance can be <= the second-best exact distance.
     float T[G];
@@ -352,36 +490,25 @@ __device__ __forceinline__ void bmu_bf16_body(const BmuModelDev &Md, const float
     int cnt[G];
 #pragma unroll
     for (int g = 0; g < G; g++) cnt[g] = 0;
-    if (nblk > 0) load_block(0, ah, al, cin);
-    for (int blk = 0; blk < nblk; blk++) {
-        bf16x8 nah[KC], nal[KC];
-        f32x16 ncin;
-        if (blk + 1 < nblk) load_block(blk + 1, nah, nal, ncin);
+    stream([&](int blk, f32x16 (&acc)[G]) {
         const int sbase_blk = blk * 32 + 4 * khalf;
 #pragma unroll
         for (int g = 0; g < G; g++) {
-            f32x16 acc = block_acc(blk, g, ah, al, cin);
             // pre-check: skip the per-value compares when no lane of the wave has a hit
             float t = __builtin_inff();
 #pragma unroll
-            for (int r = 0; r < 16; r += 2) t = fminf(fminf(t, acc[r]), acc[r + 1]);
+            for (int r = 0; r < 16; r += 2) t = fminf(fminf(t, acc[g][r]), acc[g][r + 1]);
             if (__builtin_amdgcn_ballot_w64(t <= T[g]) != 0ull) {
 #pragma unroll
                 for (int r = 0; r < 16; r++) {
-                    if (acc[r] <= T[g]) {
-                        if (cnt[g] < CAPL)
-                            cw[g][cnt[g]][lane] = (unsigned short)(sbase_blk + 8 * (r >> 2) + (r & 3));
+                    if (acc[g][r] <= T[g]) {
+                        if (cnt[g] < CAPL) cw[g][cnt[g]][lane] = (unsigned short)(sbase_blk + 8 * (r >> 2) + (r & 3));
                         cnt[g]++;
                     }
                 }
             }
         }
-        if (blk + 1 < nblk) {
-#pragma unroll
-            for (int kc = 0; kc < KC; kc++) { ah[kc] = nah[kc]; al[kc] = nal[kc]; }
-            cin = ncin;
-        }
-    }
+    });
 
     // ---------------- exact rerank of the candidate set ----------------
 #pragma unroll
@@ -406,7 +533,7 @@ __device__ __forceinline__ void bmu_bf16_body(const BmuModelDev &Md, const float
                 tp.push(sqd_mode<MODE>(wr, xr, DP), j);
             }
         }
-        // merge with partner lane
+        // merge with the partner lane
         const double pd1 = xchg32d(tp.d1), pd2 = xchg32d(tp.d2);
         const int pi1 = xchg32i(tp.i1), pi2 = xchg32i(tp.i2);
         if (pi1 >= 0) tp.push(pd1, pi1);
@@ -427,35 +554,32 @@ __device__ __forceinline__ void bmu_bf16_body(const BmuModelDev &Md, const float
 }
 
 template <int DP, int G, int MODE, bool NEED_C2>
-__global__ __launch_bounds__(BMU_WAVES * WAVE) void bmu_bf16_kernel(BmuModelDev Md, const float *__restrict__ x,
-                                                                   int64_t n, int default_index,
-                                                                   int32_t *__restrict__ c1, int32_t *__restrict__ c2,
-                                                                   double *__restrict__ d1,
-                                                                   int64_t *__restrict__ fb_list,
-                                                                   unsigned int *__restrict__ fb_count)
+__global__ __launch_bounds__(NW_BF16 * WAVE) void bmu_bf16_kernel(BmuModelDev Md, const float *__restrict__ x,
+                                                                 int64_t n, int default_index,
+                                                                 int32_t *__restrict__ c1, int32_t *__restrict__ c2,
+                                                                 double *__restrict__ d1,
+                                                                 int64_t *__restrict__ fb_list,
+                                                                 unsigned int *__restrict__ fb_count,
+                                                                 float *__restrict__ xmax_part)
 {
-    __shared__ unsigned short cand[BMU_WAVES][G][CAPL][WAVE];
-    const int wave = threadIdx.x / WAVE;
-    const int64_t sbase = ((int64_t)blockIdx.x * BMU_WAVES + wave) * (G * 32);
-    bmu_bf16_body<DP, G, MODE, NEED_C2>(Md, x, n, sbase, default_index, c1, c2, d1, fb_list, fb_count, 0,
-                                        cand[wave]);
+    __shared__ Bf16Smem<DP / 16> sm;
+    bmu_bf16_body<DP, G, MODE, NEED_C2>(Md, x, n, (int64_t)blockIdx.x * SOM_TILE_ROWS, default_index, c1, c2, d1,
+                                        fb_list, fb_count, 0, sm, xmax_part);
 }
 
 // Serving form: one workgroup per tile; tiles of different requests and models share one launch.
 template <int DP, int G, int MODE, bool NEED_C2>
-__global__ __launch_bounds__(BMU_WAVES * WAVE) void bmu_bf16_tiles_kernel(const BmuModelDev *__restrict__ models,
-                                                                         const BmuTile *__restrict__ tiles,
-                                                                         int64_t *__restrict__ fb_list,
-                                                                         unsigned int *__restrict__ fb_count)
+__global__ __launch_bounds__(NW_BF16 * WAVE) void bmu_bf16_tiles_kernel(const BmuModelDev *__restrict__ models,
+                                                                       const BmuTile *__restrict__ tiles,
+                                                                       int64_t *__restrict__ fb_list,
+                                                                       unsigned int *__restrict__ fb_count)
 {
-    __shared__ unsigned short cand[BMU_WAVES][G][CAPL][WAVE];
+    __shared__ Bf16Smem<DP / 16> sm;
     const BmuTile T = tiles[blockIdx.x];
-    if (T.n <= 0) return;
+    if (T.n <= 0) return;  // uniform over the workgroup
     const BmuModelDev Md = models[T.model];
-    const int wave = threadIdx.x / WAVE;
-    const int64_t sbase = T.row0 + (int64_t)wave * (G * 32);
-    bmu_bf16_body<DP, G, MODE, NEED_C2>(Md, T.x, T.n, sbase, T.default_index, T.c1, T.c2, T.d1, fb_list, fb_count,
-                                        (int64_t)blockIdx.x << FB_TILE_SHIFT, cand[wave]);
+    bmu_bf16_body<DP, G, MODE, NEED_C2>(Md, T.x, T.n, T.row0, T.default_index, T.c1, T.c2, T.d1, fb_list, fb_count,
+                                        (int64_t)blockIdx.x << FB_TILE_SHIFT, sm, nullptr);
 }
 
 // Fallback of the serving form: exact scan for (tile, row) entries, weights read from global memory
@@ -519,7 +643,8 @@ __global__ void prep_centre_kernel(const float *__restrict__ w, int mn, int dim,
 __global__ void prep_rows_kernel(const float *__restrict__ w, int mn, int dim, int dpad, int nblk,
                                  const float *__restrict__ centre, const int *__restrict__ perm,
                                  unsigned short *__restrict__ frag_bf, float *__restrict__ frag_f32,
-                                 float *__restrict__ norms_perm, float *__restrict__ norms_nat,
+                                 float *__restrict__ wperm, float *__restrict__ norms_perm,
+                                 float *__restrict__ norms_nat,
                                  unsigned int *__restrict__ wmax_bits, unsigned int *__restrict__ nwmax_bits)
 {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;  // slot (permuted) == neuron slot (natural)
@@ -538,6 +663,7 @@ __global__ void prep_rows_kernel(const float *__restrict__ w, int mn, int dim, i
             const unsigned short l = f32_to_bf16_bits(__fsub_rn(v, bf16_bits_to_f32(h)));
             const int kc = d >> 4, half = (d & 15) >> 3, e = d & 7;
             const int lane = row + 32 * half;
+            wperm[(int64_t)p * dpad + d] = v;
             frag_bf[((((int64_t)blk * kcn + kc) * 2 + 0) * 64 + lane) * 8 + e] = h;
             frag_bf[((((int64_t)blk * kcn + kc) * 2 + 1) * 64 + lane) * 8 + e] = l;
             if (j >= 0) atomicMax(&wmax_bits[d], __builtin_bit_cast(unsigned int, fabsf(v)));
@@ -563,18 +689,24 @@ __global__ void prep_rows_kernel(const float *__restrict__ w, int mn, int dim, i
 
 // Launch helpers -----------------------------------------------------------------------------------
 
-template <int DP>
-void launch_exact(int mode, bool need_c2, hipStream_t st, const float *x, int64_t n, int dim, const int64_t *list,
-                  const unsigned int *list_count, int64_t grid_items, const float *w, int mn, int default_index,
-                  int32_t *c1, int32_t *c2, double *d1)
+int64_t exact_blocks(int64_t grid_items)
 {
     int64_t blocks = (grid_items + 255) / 256;
     if (blocks < 1) blocks = 1;
     if (blocks > 65535 * 8) blocks = 65535 * 8;
+    return blocks;
+}
+
+template <int DP>
+void launch_exact(int mode, bool need_c2, hipStream_t st, const float *x, int64_t n, int dim, const int64_t *list,
+                  const unsigned int *list_count, int64_t grid_items, const float *w, int mn, int default_index,
+                  int32_t *c1, int32_t *c2, double *d1, float *xmax_part)
+{
+    const int64_t blocks = exact_blocks(grid_items);
     dim3 grid((unsigned)blocks), block(256);
 #define SOM_EX(M, C)                                                                                            \
     hipLaunchKernelGGL((bmu_exact_kernel<DP, M, C>), grid, block, 0, st, x, n, dim, list, list_count, w, mn,  \
-                       default_index, c1, c2, d1)
+                       default_index, c1, c2, d1, xmax_part)
     if (mode == SOMHIP_RERANK_FP64) {
         if (need_c2) SOM_EX(SOMHIP_RERANK_FP64, true); else SOM_EX(SOMHIP_RERANK_FP64, false);
     } else {
@@ -584,32 +716,30 @@ void launch_exact(int mode, bool need_c2, hipStream_t st, const float *x, int64_
 }
 
 constexpr int G_FP32 = 2;
-constexpr int G_BF16 = 2;
-static_assert(BMU_WAVES * G_BF16 * 32 == SOM_TILE_ROWS, "tile rows must match the BF16 workgroup");
 
 __global__ void zero_u32_kernel(unsigned int *p) { *p = 0u; }
 
 template <int DP>
 void launch_fp32(hipStream_t st, somhip_model *M, const float *x, int64_t n, int default_index, int32_t *c1,
-                 int32_t *c2, double *d1)
+                 int32_t *c2, double *d1, float *xmax_part)
 {
     const int64_t per_block = (int64_t)BMU_WAVES * G_FP32 * 32;
     const int64_t blocks = (n + per_block - 1) / per_block;
     hipLaunchKernelGGL((bmu_fp32_kernel<DP, G_FP32>), dim3((unsigned)blocks), dim3(BMU_WAVES * WAVE), 0, st, x, n,
                        M->dim, M->centre, (const f32x4 *)M->frag_f32, (const f32x4 *)M->norms_nat, M->nblk, M->mn,
-                       default_index, c1, c2, d1);
+                       default_index, c1, c2, d1, xmax_part);
 }
 
 template <int DP>
 void launch_bf16(int mode, bool need_c2, hipStream_t st, somhip_model *M, const float *x, int64_t n,
-                 int default_index, int32_t *c1, int32_t *c2, double *d1, int64_t *fb_list, unsigned int *fb_count)
+                 int default_index, int32_t *c1, int32_t *c2, double *d1, int64_t *fb_list, unsigned int *fb_count,
+                 float *xmax_part)
 {
-    const int64_t per_block = (int64_t)BMU_WAVES * G_BF16 * 32;
-    const int64_t blocks = (n + per_block - 1) / per_block;
-    dim3 grid((unsigned)blocks), block(BMU_WAVES * WAVE);
+    const int64_t blocks = (n + SOM_TILE_ROWS - 1) / SOM_TILE_ROWS;
+    dim3 grid((unsigned)blocks), block(NW_BF16 * WAVE);
 #define SOM_BF(MO, C)                                                                                           \
     hipLaunchKernelGGL((bmu_bf16_kernel<DP, G_BF16, MO, C>), grid, block, 0, st, somhip_model_dev(M), x, n, default_index, c1, \
-                       c2, d1, fb_list, fb_count)
+                       c2, d1, fb_list, fb_count, xmax_part)
     if (mode == SOMHIP_RERANK_FP64) {
         if (need_c2) SOM_BF(SOMHIP_RERANK_FP64, true); else SOM_BF(SOMHIP_RERANK_FP64, false);
     } else {
@@ -633,6 +763,7 @@ BmuModelDev somhip_model_dev(const somhip_model *M)
     d.centre = M->centre;
     d.wmax = M->wmax;
     d.scal = M->scal;
+    d.wperm = M->wperm;
     d.nblk = M->nblk;
     d.mn = M->mn;
     d.dim = M->dim;
@@ -654,7 +785,7 @@ int somhip_launch_bmu_tiles(hipStream_t st, int dpad, int arithmetic, int need_c
         somhip_set_error("serving supports dim <= 32");
         return SOMHIP_ERR_UNSUPPORTED;
     }
-    dim3 g((unsigned)grid), b(BMU_WAVES * WAVE);
+    dim3 g((unsigned)grid), b(NW_BF16 * WAVE);
     const int fb_grid = std::max(1, std::min(grid, 1024));
 #define SOM_TL(D, MO, C)                                                                                  \
     do {                                                                                                  \
@@ -689,7 +820,7 @@ int somhip_prepare(somhip_model *M)
     SOM_HIP_TRY(hipMemsetAsync(M->scal, 0, sizeof(float) * 4, st));
     const int slots = M->nblk * 32;
     hipLaunchKernelGGL(prep_rows_kernel, dim3((slots + 255) / 256), dim3(256), 0, st, M->w, M->mn, M->dim, M->dpad,
-                       M->nblk, M->centre, M->perm, M->frag_bf, M->frag_f32, M->norms_perm, M->norms_nat,
+                       M->nblk, M->centre, M->perm, M->frag_bf, M->frag_f32, M->wperm, M->norms_perm, M->norms_nat,
                        (unsigned int *)M->wmax, (unsigned int *)M->scal);
     SOM_HIP_TRY(hipGetLastError());
     M->prepared = true;
@@ -712,8 +843,19 @@ static int fb_workspace(somhip_ctx *C, int64_t n, int64_t **list, unsigned int *
     return SOMHIP_OK;
 }
 
+int64_t somhip_bmu_xmax_rows(int precision, int dpad, int64_t n)
+{
+    if (n <= 0) return 0;
+    if (precision == SOMHIP_PREC_EXACT) return exact_blocks(n) * 4;
+    if (precision == SOMHIP_PREC_FP32) {
+        const int64_t per_block = (int64_t)BMU_WAVES * G_FP32 * 32;
+        return (n + per_block - 1) / per_block * BMU_WAVES;
+    }
+    return (n + SOM_TILE_ROWS - 1) / SOM_TILE_ROWS * NW_BF16;
+}
+
 int somhip_launch_bmu(somhip_model *M, const float *x, int64_t n, int precision, int arithmetic, int need_c2,
-                      int default_index, int32_t *c1, int32_t *c2, double *d1)
+                      int default_index, int32_t *c1, int32_t *c2, double *d1, float *xmax_part)
 {
     if (n <= 0) return SOMHIP_OK;
     hipStream_t st = M->ctx->stream;
@@ -721,10 +863,10 @@ int somhip_launch_bmu(somhip_model *M, const float *x, int64_t n, int precision,
     if (precision == SOMHIP_PREC_EXACT) {
         TimerScope ts(M->ctx, TK_BMU_EXACT);
         switch (dp) {
-            case 16: launch_exact<16>(arithmetic, need_c2, st, x, n, M->dim, nullptr, nullptr, n, M->w, M->mn, default_index, c1, c2, d1); break;
-            case 32: launch_exact<32>(arithmetic, need_c2, st, x, n, M->dim, nullptr, nullptr, n, M->w, M->mn, default_index, c1, c2, d1); break;
-            case 48: launch_exact<48>(arithmetic, need_c2, st, x, n, M->dim, nullptr, nullptr, n, M->w, M->mn, default_index, c1, c2, d1); break;
-            case 64: launch_exact<64>(arithmetic, need_c2, st, x, n, M->dim, nullptr, nullptr, n, M->w, M->mn, default_index, c1, c2, d1); break;
+            case 16: launch_exact<16>(arithmetic, need_c2, st, x, n, M->dim, nullptr, nullptr, n, M->w, M->mn, default_index, c1, c2, d1, xmax_part); break;
+            case 32: launch_exact<32>(arithmetic, need_c2, st, x, n, M->dim, nullptr, nullptr, n, M->w, M->mn, default_index, c1, c2, d1, xmax_part); break;
+            case 48: launch_exact<48>(arithmetic, need_c2, st, x, n, M->dim, nullptr, nullptr, n, M->w, M->mn, default_index, c1, c2, d1, xmax_part); break;
+            case 64: launch_exact<64>(arithmetic, need_c2, st, x, n, M->dim, nullptr, nullptr, n, M->w, M->mn, default_index, c1, c2, d1, xmax_part); break;
             default: somhip_set_error("dim > 64 is not supported"); return SOMHIP_ERR_UNSUPPORTED;
         }
         SOM_HIP_TRY(hipGetLastError());
@@ -738,8 +880,8 @@ int somhip_launch_bmu(somhip_model *M, const float *x, int64_t n, int precision,
     if (rc) return rc;
     if (precision == SOMHIP_PREC_FP32) {
         TimerScope ts(M->ctx, TK_BMU_FP32);
-        if (dp == 16) launch_fp32<16>(st, M, x, n, default_index, c1, c2, d1);
-        else launch_fp32<32>(st, M, x, n, default_index, c1, c2, d1);
+        if (dp == 16) launch_fp32<16>(st, M, x, n, default_index, c1, c2, d1, xmax_part);
+        else launch_fp32<32>(st, M, x, n, default_index, c1, c2, d1, xmax_part);
         SOM_HIP_TRY(hipGetLastError());
         return SOMHIP_OK;
     }
@@ -758,15 +900,15 @@ int somhip_launch_bmu(somhip_model *M, const float *x, int64_t n, int precision,
     SOM_HIP_TRY(hipMemsetAsync(fb_count, 0, sizeof(unsigned int), st));
     {
         TimerScope ts(M->ctx, TK_BMU_BF16);
-        if (dp == 16) launch_bf16<16>(arithmetic, need_c2, st, M, x, n, default_index, c1, c2, d1, fb_list, fb_count);
-        else launch_bf16<32>(arithmetic, need_c2, st, M, x, n, default_index, c1, c2, d1, fb_list, fb_count);
+        if (dp == 16) launch_bf16<16>(arithmetic, need_c2, st, M, x, n, default_index, c1, c2, d1, fb_list, fb_count, xmax_part);
+        else launch_bf16<32>(arithmetic, need_c2, st, M, x, n, default_index, c1, c2, d1, fb_list, fb_count, xmax_part);
         SOM_HIP_TRY(hipGetLastError());
     }
     TimerScope tf(M->ctx, TK_BMU_FALLBACK);
     // Fallback over the flagged samples: a fixed grid that reads the device-side count.
     const int64_t fb_items = std::min<int64_t>(n, (int64_t)M->ctx->num_cus * 256);
-    if (dp == 16) launch_exact<16>(arithmetic, need_c2, st, x, n, M->dim, fb_list, fb_count, fb_items, M->w, M->mn, default_index, c1, c2, d1);
-    else launch_exact<32>(arithmetic, need_c2, st, x, n, M->dim, fb_list, fb_count, fb_items, M->w, M->mn, default_index, c1, c2, d1);
+    if (dp == 16) launch_exact<16>(arithmetic, need_c2, st, x, n, M->dim, fb_list, fb_count, fb_items, M->w, M->mn, default_index, c1, c2, d1, nullptr);
+    else launch_exact<32>(arithmetic, need_c2, st, x, n, M->dim, fb_list, fb_count, fb_items, M->w, M->mn, default_index, c1, c2, d1, nullptr);
     SOM_HIP_TRY(hipGetLastError());
     return SOMHIP_OK;
 }

@@ -81,11 +81,70 @@ __global__ void shifts_kernel(const unsigned int *__restrict__ bounds, int dim, 
     shifts[k] = s;
 }
 
+// parts[b][k] = max over rows r = b, b + RB, b + 2RB, ... of parts[r][k] (b < RB); the other rows are
+// only read, so folding in place is safe.
+__global__ __launch_bounds__(256) void rowmax_fold_kernel(float *__restrict__ parts, int64_t rows, int dpad, int RB)
+{
+    const int b = blockIdx.x;
+    __shared__ float red[256];
+    const int k = threadIdx.x % dpad;
+    const int r0 = threadIdx.x / dpad;
+    const int rs = blockDim.x / dpad;
+    float m = 0.f;
+    if (r0 < rs)
+        for (int64_t r = b + (int64_t)r0 * RB; r < rows; r += (int64_t)rs * RB) m = fmaxf(m, parts[r * dpad + k]);
+    red[threadIdx.x] = m;
+    __syncthreads();
+    if ((int)threadIdx.x < dpad) {
+        float mk = 0.f;
+        for (int j = threadIdx.x; j < rs * dpad; j += dpad) mk = fmaxf(mk, red[j]);
+        parts[(int64_t)b * dpad + threadIdx.x] = mk;
+    }
+}
+
+// Same scales from the per-wave max |x_k| rows written by the BMU kernels (rows x dpad floats) and an
+// optional max rho (float bits, from absmax_kernel on rho); column dim is the rho column.
+__global__ __launch_bounds__(1024) void shifts_parts_kernel(const float *__restrict__ parts, int64_t rows, int dpad,
+                                                           int dim, const unsigned int *__restrict__ rho_bound,
+                                                           int64_t n, int *__restrict__ shifts)
+{
+    __shared__ float red[1024];
+    const int k = threadIdx.x % dpad;
+    const int r0 = threadIdx.x / dpad;
+    const int rs = blockDim.x / dpad;
+    float m = 0.f;
+    if (r0 < rs)
+        for (int64_t r = r0; r < rows; r += rs) m = fmaxf(m, parts[r * dpad + k]);
+    red[threadIdx.x] = m;
+    __syncthreads();
+    if ((int)threadIdx.x <= dim) {
+        const double rb = rho_bound ? (double)__builtin_bit_cast(float, rho_bound[0]) : 1.0;
+        double B;
+        if ((int)threadIdx.x == dim) {
+            B = rb;
+        } else {
+            float mk = 0.f;
+            for (int j = threadIdx.x; j < rs * dpad; j += dpad) mk = fmaxf(mk, red[j]);
+            B = (double)mk * rb;
+        }
+        int s = 0;
+        if (B > 0.0) {
+            int e;
+            frexp(B * (double)(n > 0 ? n : 1), &e);
+            s = 61 - e;
+            if (s > 1000) s = 1000;
+            if (s < -1000) s = -1000;
+        }
+        shifts[threadIdx.x] = s;
+    }
+}
+
 // ------------------------------------------------------------------------------------------------
 // Fixed-point scatter of per-sample rows into per-neuron integer sums.
 //   nv = dim + 1 columns (x != NULL: rho*x_0..rho*x_{dim-1}, rho) or nv = 1 (x == NULL: rho).
 //   Samples with bmu < 0 or !(rho > 0) are skipped. rho_f32: round rho to float first (training).
 // ------------------------------------------------------------------------------------------------
+template <int DP>
 __global__ __launch_bounds__(512) void scatter_fixed_kernel(const int32_t *__restrict__ bmu,
                                                             const float *__restrict__ x, int dim,
                                                             const double *__restrict__ rho, int rho_f32, int64_t n,
@@ -93,30 +152,50 @@ __global__ __launch_bounds__(512) void scatter_fixed_kernel(const int32_t *__res
                                                             int nv, int slice_len,
                                                             unsigned long long *__restrict__ partial)
 {
+    constexpr int U = 4;  // samples in flight per thread
     extern __shared__ unsigned long long lds[];
     const int64_t i0 = (int64_t)blockIdx.x * per_wg;
     const int64_t i1 = min(n, i0 + per_wg);
     unsigned long long *out = partial + (int64_t)blockIdx.x * mn * nv;
+    double scale[DP];
+#pragma unroll
+    for (int k = 0; k < DP; k++) scale[k] = (k < nv - 1) ? ldexp(1.0, shifts[k]) : 0.0;
+    const double srho = ldexp(1.0, shifts[nv - 1]);
     for (int c0 = 0; c0 < mn; c0 += slice_len) {
         const int cl = min(slice_len, mn - c0);
         for (int e = threadIdx.x; e < cl * nv; e += blockDim.x) lds[e] = 0ull;
         __syncthreads();
-        for (int64_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
-            const int c = bmu[i];
-            if (c < c0 || c >= c0 + cl) continue;
-            double r = rho ? rho[i] : 1.0;
-            if (rho_f32) r = (double)(float)r;
-            if (!(r > 0.0) || !(r < 1e300)) continue;
-            unsigned long long *row = lds + (int64_t)(c - c0) * nv;
-            if (x) {
-                for (int k = 0; k < dim; k++) {
-                    const double v = r * (double)x[i * dim + k];
-                    const long long q = llrint(ldexp(v, shifts[k]));
-                    atomicAdd(&row[k], (unsigned long long)q);
-                }
+        for (int64_t ib = i0 + threadIdx.x; ib < i1; ib += (int64_t)U * blockDim.x) {
+            int cc[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int64_t i = ib + (int64_t)u * blockDim.x;
+                const int c = i < i1 ? bmu[i] : -1;
+                cc[u] = (c >= c0 && c < c0 + cl) ? c - c0 : -1;
             }
-            const long long qr = llrint(ldexp(r, shifts[nv - 1]));
-            atomicAdd(&row[nv - 1], (unsigned long long)qr);
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                if (cc[u] < 0) continue;
+                const int64_t i = ib + (int64_t)u * blockDim.x;
+                double r = rho ? rho[i] : 1.0;
+                if (rho_f32) r = (double)(float)r;
+                if (!(r > 0.0) || !(r < 1e300)) continue;
+                unsigned long long *row = lds + (int64_t)cc[u] * nv;
+                if (x) {
+                    float xv[DP];
+#pragma unroll
+                    for (int k = 0; k < DP; k++) xv[k] = k < dim ? x[i * dim + k] : 0.f;
+#pragma unroll
+                    for (int k = 0; k < DP; k++) {
+                        if (k < dim) {
+                            const long long q = llrint(r * (double)xv[k] * scale[k]);
+                            atomicAdd(&row[k], (unsigned long long)q);
+                        }
+                    }
+                }
+                const long long qr = llrint(r * srho);
+                atomicAdd(&row[nv - 1], (unsigned long long)qr);
+            }
         }
         __syncthreads();
         for (int e = threadIdx.x; e < cl * nv; e += blockDim.x) out[(int64_t)c0 * nv + e] = lds[e];
@@ -347,6 +426,21 @@ int somhip_launch_shifts(hipStream_t st, const unsigned int *bounds, int dim, in
     return SOMHIP_OK;
 }
 
+int somhip_launch_shifts_parts(hipStream_t st, const float *parts, int64_t rows, int dpad, int dim,
+                               const unsigned int *rho_bound, int64_t n, int *shifts)
+{
+    // first stage: fold the rows in place into the first RB rows (RB x dpad maxima), then one block
+    constexpr int RB = 256;
+    if (rows > RB) {
+        hipLaunchKernelGGL(rowmax_fold_kernel, dim3(RB), dim3(256), 0, st, (float *)parts, rows, dpad, RB);
+        SOM_HIP_TRY(hipGetLastError());
+        rows = RB;
+    }
+    hipLaunchKernelGGL(shifts_parts_kernel, dim3(1), dim3(1024), 0, st, parts, rows, dpad, dim, rho_bound, n, shifts);
+    SOM_HIP_TRY(hipGetLastError());
+    return SOMHIP_OK;
+}
+
 // Number of scatter workgroups so that the partial table stays within `table_budget` bytes.
 int somhip_scatter_nwg(int64_t n, int mn, int nv, int num_cus, size_t table_budget)
 {
@@ -369,11 +463,20 @@ int somhip_launch_scatter(hipStream_t st, const int32_t *bmu, const float *x, in
     }
     if (slice_len > mn) slice_len = mn;
     const size_t lds = (size_t)slice_len * nv * 8;
-    SOM_HIP_TRY(hipFuncSetAttribute((const void *)scatter_fixed_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    (int)lds));
     const int64_t per_wg = (n + nwg - 1) / nwg;
-    hipLaunchKernelGGL(scatter_fixed_kernel, dim3(nwg), dim3(512), lds, st, bmu, x, dim, rho, rho_f32, n, per_wg,
-                       shifts, mn, nv, slice_len, partial);
+    const int dpad = (dim + 15) / 16 * 16;
+#define SOM_SC(D)                                                                                              \
+    do {                                                                                                       \
+        SOM_HIP_TRY(hipFuncSetAttribute((const void *)scatter_fixed_kernel<D>,                                 \
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));                \
+        hipLaunchKernelGGL(scatter_fixed_kernel<D>, dim3(nwg), dim3(512), lds, st, bmu, x, dim, rho, rho_f32, n, \
+                           per_wg, shifts, mn, nv, slice_len, partial);                                        \
+    } while (0)
+    if (!x || dpad <= 16) SOM_SC(16);
+    else if (dpad == 32) SOM_SC(32);
+    else if (dpad == 48) SOM_SC(48);
+    else SOM_SC(64);
+#undef SOM_SC
     SOM_HIP_TRY(hipGetLastError());
     const int64_t width = (int64_t)mn * nv;
     hipLaunchKernelGGL(scatter_reduce_kernel, dim3((unsigned)((width + 255) / 256)), dim3(256), 0, st, partial, nwg,

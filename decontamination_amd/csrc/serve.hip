@@ -4,7 +4,7 @@
 //   submit()  -> pending queue (FIFO)
 //   admission -> a request is admitted when its working memory fits in the free part of the budget
 //                (one device slab, first-fit); its rows are uploaded on the copy stream
-//   batching  -> every iteration, the tiles (256 rows) of all uploaded requests are shared out
+//   batching  -> every iteration, the tiles (512 rows) of all uploaded requests are shared out
 //                round-robin into one launch per dimension class; the launch is a hipGraph captured
 //                per (dimension, tile-count bucket) with a fixed tile-table buffer, so a replay is a
 //                table upload plus one graph launch
