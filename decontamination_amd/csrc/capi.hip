@@ -58,34 +58,69 @@ inline int check_prec(int precision, int arithmetic)
     return SOMHIP_OK;
 }
 
-// Choose a slot permutation for the BF16 path: slot p -> neuron (p*S) mod mn, with S coprime to mn
-// and no multiple k*S (k < 32) landing on a grid neighbour offset, so that a neuron and its grid
-// neighbours never share a 32-slot block. Padding slots map to -1.
-std::vector<int> make_perm(int m, int n, int nblk)
+// Slot order of the BF16 path.
+//
+// Base order: slot p -> neuron (p*S) mod mn, S coprime to mn and chosen so that no multiple k*S
+// (k < KSPREAD) lands on a grid neighbour offset (+-1, +-n, +-n+-1): a neuron and its grid
+// neighbours then sit in different 32-slot blocks, which keeps the second of the per-block minima
+// close to the true second-best distance.
+//
+// Coarse section: on maps of at least 1024 neurons the sub-lattice {(a, b) : a % 3 == 0, b % 3 == 0}
+// (neuron a*n + b) goes first, in base order, padded to a whole number of 8-block chunks; the other
+// neurons follow in base order. The BMU kernel scans the coarse section first to get a threshold
+// before it scans the rest. Padding slots map to -1.
+struct PermLayout {
+    std::vector<int> perm;
+    int pblk = 0, pblk_a = 0;
+};
+
+int spread_stride(int m, int n, int kspread)
 {
     const int mn = m * n;
-    std::vector<int> perm((size_t)nblk * 32, -1);
-    int S = 1;
-    if (mn > 64) {
-        auto bad = [&](int s) {
-            if (std::gcd(s, mn) != 1) return true;
-            for (int k = 1; k < 32; k++) {
-                long long o = ((long long)k * s) % mn;
-                long long d = std::min(o, (long long)mn - o);
-                if (d <= 1 || std::llabs(d - n) <= 1) return true;
-            }
-            return false;
-        };
-        const int start = std::max(2, (int)(0.6180339887 * mn));
-        S = 0;
-        for (int s = start; s < mn && !S; s++)
-            if (!bad(s)) S = s;
-        for (int s = start - 1; s > 1 && !S; s--)
-            if (!bad(s)) S = s;
-        if (!S) S = 1;
+    if (mn <= 64) return 1;
+    auto bad = [&](int s) {
+        if (std::gcd(s, mn) != 1) return true;
+        for (int k = 1; k < kspread; k++) {
+            long long o = ((long long)k * s) % mn;
+            long long d = std::min(o, (long long)mn - o);
+            if (d <= 1 || std::llabs(d - n) <= 1) return true;
+        }
+        return false;
+    };
+    const int start = std::max(2, (int)(0.6180339887 * mn));
+    for (int s = start; s < mn; s++)
+        if (!bad(s)) return s;
+    for (int s = start - 1; s > 1; s--)
+        if (!bad(s)) return s;
+    return 0;
+}
+
+PermLayout make_perm(int m, int n)
+{
+    const int mn = m * n;
+    int S = spread_stride(m, n, 40);
+    if (!S) S = spread_stride(m, n, 32);
+    if (!S) S = 1;
+    std::vector<int> base(mn);
+    for (int p = 0; p < mn; p++) base[p] = (int)(((long long)p * S) % mn);
+    PermLayout L;
+    constexpr int COARSE_STEP = 3, CHUNK_BLOCKS = 8;
+    if (mn >= 1024) {
+        std::vector<int> coarse, rest;
+        for (int j : base) ((j / n) % COARSE_STEP == 0 && (j % n) % COARSE_STEP == 0 ? coarse : rest).push_back(j);
+        const int na = ((int)coarse.size() + 32 * CHUNK_BLOCKS - 1) / (32 * CHUNK_BLOCKS) * CHUNK_BLOCKS;
+        const int nr = ((int)rest.size() + 31) / 32;
+        L.pblk_a = na;
+        L.pblk = na + nr;
+        L.perm.assign((size_t)L.pblk * 32, -1);
+        std::copy(coarse.begin(), coarse.end(), L.perm.begin());
+        std::copy(rest.begin(), rest.end(), L.perm.begin() + (size_t)na * 32);
+    } else {
+        L.pblk = L.pblk_a = (mn + 31) / 32;
+        L.perm.assign((size_t)L.pblk * 32, -1);
+        std::copy(base.begin(), base.end(), L.perm.begin());
     }
-    for (int p = 0; p < mn; p++) perm[p] = (int)(((long long)p * S) % mn);
-    return perm;
+    return L;
 }
 
 }  // namespace
@@ -335,7 +370,11 @@ int somhip_model_create(somhip_ctx *C, int m, int n, int dim, int topology, cons
     M->dpad = (dim + 15) / 16 * 16;
     M->topology = topology;
     M->nblk = (M->mn + 31) / 32;
+    PermLayout layout = make_perm(m, n);
+    M->pblk = layout.pblk;
+    M->pblk_a = layout.pblk_a;
     const size_t slots = (size_t)M->nblk * 32;
+    const size_t pslots = (size_t)M->pblk * 32;
     auto fail = [&](hipError_t e, const char *what) {
         somhip_set_error(std::string(what) + ": " + hipGetErrorString(e));
         somhip_model_destroy(M);
@@ -350,15 +389,14 @@ int somhip_model_create(somhip_ctx *C, int m, int n, int dim, int topology, cons
     SOM_ALLOC(M->centre, sizeof(float) * M->dpad);
     SOM_ALLOC(M->wmax, sizeof(float) * M->dpad);
     SOM_ALLOC(M->scal, sizeof(float) * 4);
-    SOM_ALLOC(M->norms_perm, sizeof(float) * slots);
+    SOM_ALLOC(M->norms_perm, sizeof(float) * pslots);
     SOM_ALLOC(M->norms_nat, sizeof(float) * slots);
-    SOM_ALLOC(M->frag_bf, sizeof(unsigned short) * slots * M->dpad * 2);
+    SOM_ALLOC(M->frag_bf, sizeof(unsigned short) * pslots * M->dpad * 2);
     SOM_ALLOC(M->frag_f32, sizeof(float) * slots * M->dpad);
-    SOM_ALLOC(M->wperm, sizeof(float) * slots * M->dpad);
-    SOM_ALLOC(M->perm, sizeof(int) * slots);
+    SOM_ALLOC(M->wperm, sizeof(float) * pslots * M->dpad);
+    SOM_ALLOC(M->perm, sizeof(int) * pslots);
 #undef SOM_ALLOC
-    std::vector<int> perm = make_perm(m, n, M->nblk);
-    if ((e = hipMemcpyAsync(M->perm, perm.data(), sizeof(int) * slots, hipMemcpyHostToDevice, C->stream)) != hipSuccess)
+    if ((e = hipMemcpyAsync(M->perm, layout.perm.data(), sizeof(int) * pslots, hipMemcpyHostToDevice, C->stream)) != hipSuccess)
         return fail(e, "hipMemcpy perm");
     if ((e = hipMemcpyAsync(M->topo, topo_host, sizeof(float) * M->mn * 2, hipMemcpyHostToDevice, C->stream)) != hipSuccess)
         return fail(e, "hipMemcpy topography");

@@ -106,7 +106,9 @@ struct BmuModelDev {
     const float *wmax;      // max_j |w'_jk| (dpad)
     const float *scal;      // [0] max_j |w'_j|^2
     const float *wperm;     // FP32 centred weights per permuted slot, (slots, dpad) row-major
-    int nblk, mn, dim;
+    int nblk;               // 32-slot blocks of the permuted order
+    int nblk_a;             // leading blocks that hold the coarse sub-lattice (== nblk: none)
+    int mn, dim;
 };
 
 // One unit of serving work: rows [row0, row0 + SOM_TILE_ROWS) of one request.
@@ -205,7 +207,9 @@ struct somhip_ctx {
 struct somhip_model {
     somhip_ctx *ctx = nullptr;
     int m = 0, n = 0, mn = 0, dim = 0, dpad = 0, topology = SOMHIP_TOPOLOGY_HEXAGONAL;
-    int nblk = 0;          // number of 32-neuron blocks (mn rounded up to 32)
+    int nblk = 0;          // number of 32-neuron blocks in natural order (mn rounded up to 32)
+    int pblk = 0;          // number of 32-slot blocks in the permuted (BF16) order
+    int pblk_a = 0;        // leading permuted blocks holding the coarse sub-lattice (== pblk: none)
     float *w = nullptr;    // (mn, dim) FP32 weights, row-major
     float *topo = nullptr; // (mn, 2) FP32 topography
     // prepared data (valid when prepared == true)
@@ -213,12 +217,12 @@ struct somhip_model {
     float *centre = nullptr;        // (dpad) FP32 centroid of the weights
     float *wmax = nullptr;          // (dpad) max_j |w'_jk|
     float *scal = nullptr;          // [0] = max_j |w'_j|^2 (as float)
-    float *norms_perm = nullptr;    // (nblk*32) |w'|^2 in permuted slot order (padding slots = +inf)
+    float *norms_perm = nullptr;    // (pblk*32) |w'|^2 in permuted slot order (padding slots = +inf)
     float *norms_nat = nullptr;     // (nblk*32) |w'|^2 in natural order (padding slots = +inf)
     unsigned short *frag_bf = nullptr;  // bf16 A fragments [blk][kc][hi,lo][64 lanes][8]
     float *frag_f32 = nullptr;      // FP32 A fragments [blk][dpad/8][64 lanes][4]
     float *wperm = nullptr;         // FP32 centred weights per permuted slot [slot][dpad]
-    int *perm = nullptr;            // (nblk*32) slot -> neuron (or -1 for padding)
+    int *perm = nullptr;            // (pblk*32) slot -> neuron (or -1 for padding)
 };
 
 // Timing helpers: bracket a launch with timer_begin / timer_end (no-ops unless enabled).

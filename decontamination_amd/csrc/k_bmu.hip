@@ -11,10 +11,12 @@
 //   FP32        bmu_fp32_kernel: |w'|^2 - 2 x'.w' on v_mfma_f32_32x32x2_f32 (x', w' centred on the
 //               weight centroid), top-2 on those values (the plain FP32 form).
 //   BF16_RERANK bmu_bf16_kernel: the same GEMM on v_mfma_f32_32x32x16_bf16 with a 3-term split
-//               (x_h w_h + x_l w_h + x_h w_l), pass 1 = top-2 of per-lane block minima, pass 2 =
-//               every neuron inside a provable error window of the second minimum, then the exact
-//               distance formula on that candidate set. Samples whose window overflows the
-//               per-lane list, or whose bound is not finite, go to the EXACT fallback.
+//               (x_h w_h + x_l w_h + x_h w_l). Phase A scans a coarse sub-lattice of the map for the
+//               top-2 of per-lane block minima; phase B scans the other neurons, recording every
+//               neuron inside a provable error window of the running second minimum while it
+//               tightens the window; phase C records the coarse neurons under the final window.
+//               Then the exact distance formula on the recorded set. Samples whose list overflows,
+//               or whose bound is not finite, go to the EXACT fallback.
 #include "common.hpp"
 
 #include <algorithm>
@@ -312,7 +314,7 @@ struct Bf16Smem {
     static constexpr int CB = CHUNK_KB / KC;  // blocks per chunk
     uint4 frag[2][FRAG_UNITS];          // [buf][cb][kc][hi,lo][lane] 16 B
     float norms[2][CB * 32];            // [buf][cb][row]
-    unsigned short cand[NW_BF16][G_BF16][CAPL][WAVE];
+    unsigned short cand[NW_BF16][G_BF16][CAPL + 1][WAVE];  // last row: writes past a full list
 };
 
 template <int DP, int G, int MODE, bool NEED_C2>
@@ -342,7 +344,7 @@ __device__ __forceinline__ void bmu_bf16_body(const BmuModelDev &Md, const float
     const int col = lane & 31;
     const int khalf = lane >> 5;
     const int64_t sbase = tile_row0 + (int64_t)wave * (G * 32);
-    unsigned short(*cw)[CAPL][WAVE] = sm.cand[wave];
+    unsigned short(*cw)[CAPL + 1][WAVE] = sm.cand[wave];
     const int nchunks = (nblk + CB - 1) / CB;
     const int64_t frag_units_total = (int64_t)nblk * KC * 2 * 64;
     const int64_t norm_units_total = (int64_t)nblk * 8;
@@ -433,16 +435,16 @@ __device__ __forceinline__ void bmu_bf16_body(const BmuModelDev &Md, const float
         }
         return acc;
     };
-    auto stream = [&](auto &&per_block) {
-        __syncthreads();  // the buffers may still be read by the previous pass
-        if (nchunks > 0) {
-            load_chunk(0);
-            store_chunk(0);
-        }
+    // Streams chunks [c0, c1) through the LDS buffers; per_block(blk, acc) sees every block's D values,
+    // after_chunk() runs once per chunk, after the chunk's closing barrier.
+    auto stream = [&](int c0, int c1, auto &&per_block, auto &&after_chunk) {
+        __syncthreads();  // the buffers may still be read by the previous phase
+        load_chunk(c0);
+        store_chunk(0);
         __syncthreads();
-        for (int c = 0; c < nchunks; c++) {
-            const int buf = c & 1;
-            if (c + 1 < nchunks) load_chunk(c + 1);
+        for (int c = c0; c < c1; c++) {
+            const int buf = (c - c0) & 1;
+            if (c + 1 < c1) load_chunk(c + 1);
             const int b0 = c * CB;
             const int b1 = min(nblk, b0 + CB);
             for (int blk = b0; blk < b1; blk++) {
@@ -454,61 +456,94 @@ __device__ __forceinline__ void bmu_bf16_body(const BmuModelDev &Md, const float
                 for (int g = 0; g < G; g++) acc[g] = block_acc(g, ah, al, cin);
                 per_block(blk, acc);
             }
-            if (c + 1 < nchunks) store_chunk(buf ^ 1);
+            if (c + 1 < c1) store_chunk(buf ^ 1);
             __syncthreads();
+            after_chunk();
         }
     };
+    auto block_min = [](const f32x16 &v) {
+        float t = __builtin_inff();
+#pragma unroll
+        for (int r = 0; r < 16; r += 2) t = fminf(fminf(t, v[r]), v[r + 1]);
+        return t;
+    };
 
-    // ---------------- pass 1: top-2 of per-lane block minima ----------------
-    float m1[G], m2[G];
+    // Own top-2 of this lane's half-block minima. The two lanes of a sample cover disjoint neuron
+    // sets, so the merged top-2 are approximate distances of two distinct neurons.
+    float m1[G], m2[G], T[G];
 #pragma unroll
     for (int g = 0; g < G; g++) m1[g] = m2[g] = __builtin_inff();
-    stream([&](int blk, f32x16 (&acc)[G]) {
+    // Threshold from the merged top-2: every neuron whose exact distance can be <= the second-best
+    // (best, without NEED_C2) exact distance has D <= T. T only decreases as more blocks are merged.
+    auto refresh = [&]() {
 #pragma unroll
         for (int g = 0; g < G; g++) {
-            float t = __builtin_inff();
-#pragma unroll
-            for (int r = 0; r < 16; r += 2) t = fminf(fminf(t, acc[g][r]), acc[g][r + 1]);
-            m2[g] = __builtin_amdgcn_fmed3f(m1[g], m2[g], t);
-            m1[g] = fminf(m1[g], t);
+            const float p1 = xchg32(m1[g]), p2 = xchg32(m2[g]);
+            const float A1 = fminf(m1[g], p1);
+            const float A2 = fminf(fmaxf(m1[g], p1), fminf(m2[g], p2));
+            const float A = NEED_C2 ? A2 : A1;
+            const float scale = fmaxf(A + K[g] + E[g], 0.f);
+            T[g] = A + 2.f * E[g] + 6.2f * U32 * scale + 8.f * U32 * fabsf(A);
         }
-    });
-
-    // Threshold: every neuron whose exact distance can be <= the second-best exact distance.
-    float T[G];
-#pragma unroll
-    for (int g = 0; g < G; g++) {
-        const float p1 = xchg32(m1[g]), p2 = xchg32(m2[g]);
-        const float A1 = fminf(m1[g], p1);
-        const float A2 = fminf(fmaxf(m1[g], p1), fminf(m2[g], p2));
-        const float A = NEED_C2 ? A2 : A1;
-        const float scale = fmaxf(A + K[g] + E[g], 0.f);
-        T[g] = A + 2.f * E[g] + 6.2f * U32 * scale + 8.f * U32 * fabsf(A);
-    }
-
-    // ---------------- pass 2: collect the window ----------------
+    };
+    auto update = [&](int g, float t) {
+        m2[g] = __builtin_amdgcn_fmed3f(m1[g], m2[g], t);
+        m1[g] = fminf(m1[g], t);
+    };
     int cnt[G];
 #pragma unroll
     for (int g = 0; g < G; g++) cnt[g] = 0;
-    stream([&](int blk, f32x16 (&acc)[G]) {
-        const int sbase_blk = blk * 32 + 4 * khalf;
+    // Appends every row of this lane's half-block with D <= T; the pre-check skips the per-value
+    // compares when no lane of the wave has a hit in the block.
+    // The 16 row compares are issued back to back as wave masks; rows without any hit cost one scalar
+    // test. A full list keeps counting (overflow sends the sample to the fallback) and writes to the
+    // spare last row.
+    auto record = [&](int blk, int g, const f32x16 &v, float t) {
+        if (__builtin_amdgcn_ballot_w64(t <= T[g]) != 0ull) {
+            const int sbase_blk = blk * 32 + 4 * khalf;
+            unsigned long long bm[16];
 #pragma unroll
-        for (int g = 0; g < G; g++) {
-            // pre-check: skip the per-value compares when no lane of the wave has a hit
-            float t = __builtin_inff();
+            for (int r = 0; r < 16; r++) bm[r] = __builtin_amdgcn_ballot_w64(v[r] <= T[g]);
 #pragma unroll
-            for (int r = 0; r < 16; r += 2) t = fminf(fminf(t, acc[g][r]), acc[g][r + 1]);
-            if (__builtin_amdgcn_ballot_w64(t <= T[g]) != 0ull) {
-#pragma unroll
-                for (int r = 0; r < 16; r++) {
-                    if (acc[g][r] <= T[g]) {
-                        if (cnt[g] < CAPL) cw[g][cnt[g]][lane] = (unsigned short)(sbase_blk + 8 * (r >> 2) + (r & 3));
-                        cnt[g]++;
-                    }
+            for (int r = 0; r < 16; r++) {
+                if (bm[r] != 0ull) {  // uniform; lanes without a hit write to the spare row
+                    const bool hit = v[r] <= T[g];
+                    cw[g][hit ? min(cnt[g], CAPL) : CAPL][lane] = (unsigned short)(sbase_blk + 8 * (r >> 2) + (r & 3));
+                    cnt[g] += hit ? 1 : 0;
                 }
             }
         }
-    });
+    };
+    auto nothing = [] {};
+    const int nchunks_a = (Md.nblk_a + CB - 1) / CB;
+
+    // ---------------- phase A: coarse section, top-2 only ----------------
+    stream(0, nchunks_a, [&](int, f32x16 (&acc)[G]) {
+#pragma unroll
+        for (int g = 0; g < G; g++) update(g, block_min(acc[g]));
+    }, nothing);
+    refresh();
+
+    // ---------------- phase B: the other blocks, recording under a running threshold ----------------
+    // A block is recorded against the threshold in force before its own minima are merged; the
+    // threshold in force at any time is >= the final one, so the recorded set contains every
+    // neuron with D <= final T.
+    if (nchunks_a < nchunks) {
+        stream(nchunks_a, nchunks, [&](int blk, f32x16 (&acc)[G]) {
+#pragma unroll
+            for (int g = 0; g < G; g++) {
+                const float t = block_min(acc[g]);
+                record(blk, g, acc[g], t);
+                update(g, t);
+            }
+        }, refresh);
+    }
+
+    // ---------------- phase C: the coarse section again, recording under the final threshold ----------------
+    stream(0, nchunks_a, [&](int blk, f32x16 (&acc)[G]) {
+#pragma unroll
+        for (int g = 0; g < G; g++) record(blk, g, acc[g], block_min(acc[g]));
+    }, nothing);
 
     // ---------------- exact rerank of the candidate set ----------------
 #pragma unroll
@@ -582,41 +617,78 @@ __global__ __launch_bounds__(NW_BF16 * WAVE) void bmu_bf16_tiles_kernel(const Bm
                                         (int64_t)blockIdx.x << FB_TILE_SHIFT, sm, nullptr);
 }
 
-// Fallback of the serving form: exact scan for (tile, row) entries, weights read from global memory
-// (entries of one launch may belong to different models).
-template <int DP, int MODE, bool NEED_C2>
+// Exact top-2 of one sample by one wave: lane l scans neurons l, l+64, ...; the lanes' disjoint
+// top-2 lists are merged by a butterfly, lexicographic on (distance, index).
+template <int DP, int MODE>
+__device__ __forceinline__ Top2 wave_exact_top2(const float *__restrict__ xrow, int dim, const float *__restrict__ w,
+                                                int mn, int lane)
+{
+    float xr[DP];
+#pragma unroll
+    for (int k = 0; k < DP; k++) xr[k] = k < dim ? xrow[k] : 0.f;
+    Top2 tp;
+    tp.init();
+    for (int j = lane; j < mn; j += WAVE) {
+        float wr[DP];
+#pragma unroll
+        for (int k = 0; k < DP; k++) wr[k] = k < dim ? w[(int64_t)j * dim + k] : 0.f;
+        tp.push(sqd_mode<MODE>(wr, xr, DP), j);
+    }
+#pragma unroll
+    for (int o = WAVE / 2; o > 0; o >>= 1) {
+        const double pd1 = __shfl_xor(tp.d1, o, WAVE), pd2 = __shfl_xor(tp.d2, o, WAVE);
+        const int pi1 = __shfl_xor(tp.i1, o, WAVE), pi2 = __shfl_xor(tp.i2, o, WAVE);
+        if (pi1 >= 0) tp.push(pd1, pi1);
+        if (pi2 >= 0) tp.push(pd2, pi2);
+    }
+    return tp;
+}
+
+// Fallback of the plain BF16 form: one wave per listed sample, grid-stride over the device-side count.
+template <int DP, int MODE>
+__global__ __launch_bounds__(256) void bmu_exact_wave_kernel(const float *__restrict__ x, int dim,
+                                                             const int64_t *__restrict__ list,
+                                                             const unsigned int *__restrict__ list_count,
+                                                             const float *__restrict__ w, int mn, int default_index,
+                                                             int32_t *__restrict__ c1, int32_t *__restrict__ c2,
+                                                             double *__restrict__ d1)
+{
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int64_t nw = (int64_t)gridDim.x * (blockDim.x / WAVE);
+    const int64_t total = (int64_t)(*list_count);
+    for (int64_t t = (int64_t)blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE; t < total; t += nw) {
+        const int64_t i = list[t];
+        const Top2 tp = wave_exact_top2<DP, MODE>(x + i * dim, dim, w, mn, lane);
+        if (lane == 0) {
+            c1[i] = tp.i1 >= 0 ? tp.i1 : default_index;
+            if (c2) c2[i] = tp.i2;
+            if (d1) d1[i] = tp.i1 >= 0 ? tp.d1 : __builtin_nan("");
+        }
+    }
+}
+
+// Fallback of the serving form: one wave per (tile, row) entry of the list (entries of one launch
+// may belong to different models).
+template <int DP, int MODE>
 __global__ __launch_bounds__(256) void bmu_exact_tiles_kernel(const BmuModelDev *__restrict__ models,
                                                              const BmuTile *__restrict__ tiles,
                                                              const int64_t *__restrict__ list,
                                                              const unsigned int *__restrict__ list_count)
 {
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int64_t nw = (int64_t)gridDim.x * (blockDim.x / WAVE);
     const int64_t total = (int64_t)(*list_count);
-    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+    for (int64_t t = (int64_t)blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE; t < total; t += nw) {
         const int64_t e = list[t];
         const BmuTile T = tiles[e >> FB_TILE_SHIFT];
         const int64_t i = e & ((1ll << FB_TILE_SHIFT) - 1);
         const BmuModelDev Md = models[T.model];
-        const int dim = Md.dim;
-        float xr[DP];
-#pragma unroll
-        for (int k = 0; k < DP; k++) xr[k] = k < dim ? T.x[i * dim + k] : 0.f;
-        double md1 = 1.0e99, md2 = 1.0e99;
-        int mi1 = -1, mi2 = -1;
-        for (int j = 0; j < Md.mn; j++) {
-            float wr[DP];
-#pragma unroll
-            for (int k = 0; k < DP; k++) wr[k] = k < dim ? Md.w[(int64_t)j * dim + k] : 0.f;
-            const double d = sqd_mode<MODE>(wr, xr, DP);
-            if (d != d) continue;
-            if (md1 > d) {
-                md2 = md1; mi2 = mi1; md1 = d; mi1 = j;
-            } else if (NEED_C2 && md2 > d) {
-                md2 = d; mi2 = j;
-            }
+        const Top2 tp = wave_exact_top2<DP, MODE>(T.x + i * Md.dim, Md.dim, Md.w, Md.mn, lane);
+        if (lane == 0) {
+            T.c1[i] = tp.i1 >= 0 ? tp.i1 : T.default_index;
+            if (T.c2) T.c2[i] = tp.i2;
+            if (T.d1) T.d1[i] = tp.i1 >= 0 ? tp.d1 : __builtin_nan("");
         }
-        T.c1[i] = mi1 >= 0 ? mi1 : T.default_index;
-        if (T.c2) T.c2[i] = mi2;
-        if (T.d1) T.d1[i] = mi1 >= 0 ? md1 : __builtin_nan("");
     }
 }
 
@@ -640,7 +712,7 @@ __global__ void prep_centre_kernel(const float *__restrict__ w, int mn, int dim,
     if (threadIdx.x == 0) centre[k] = k < dim ? (float)(red[0] / (double)mn) : 0.f;
 }
 
-__global__ void prep_rows_kernel(const float *__restrict__ w, int mn, int dim, int dpad, int nblk,
+__global__ void prep_rows_kernel(const float *__restrict__ w, int mn, int dim, int dpad, int nblk, int pblk,
                                  const float *__restrict__ centre, const int *__restrict__ perm,
                                  unsigned short *__restrict__ frag_bf, float *__restrict__ frag_f32,
                                  float *__restrict__ wperm, float *__restrict__ norms_perm,
@@ -648,11 +720,10 @@ __global__ void prep_rows_kernel(const float *__restrict__ w, int mn, int dim, i
                                  unsigned int *__restrict__ wmax_bits, unsigned int *__restrict__ nwmax_bits)
 {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;  // slot (permuted) == neuron slot (natural)
-    if (p >= nblk * 32) return;
     const int kcn = dpad / 16;
     const int ks4 = dpad / 8;
     // permuted slot -> bf16 fragments and permuted norms
-    {
+    if (p < pblk * 32) {
         const int j = perm[p];
         float nrm = 0.f;
         const int blk = p >> 5, row = p & 31;
@@ -672,7 +743,7 @@ __global__ void prep_rows_kernel(const float *__restrict__ w, int mn, int dim, i
         if (j >= 0) atomicMax(nwmax_bits, __builtin_bit_cast(unsigned int, nrm));
     }
     // natural slot -> FP32 fragments and natural norms
-    {
+    if (p < nblk * 32) {
         const int j = p < mn ? p : -1;
         float nrm = 0.f;
         const int blk = p >> 5, row = p & 31;
@@ -713,6 +784,19 @@ void launch_exact(int mode, bool need_c2, hipStream_t st, const float *x, int64_
         if (need_c2) SOM_EX(SOMHIP_RERANK_REF32, true); else SOM_EX(SOMHIP_RERANK_REF32, false);
     }
 #undef SOM_EX
+}
+
+template <int DP>
+void launch_exact_wave(int mode, hipStream_t st, const float *x, int dim, const int64_t *list,
+                       const unsigned int *list_count, int64_t blocks, const float *w, int mn, int default_index,
+                       int32_t *c1, int32_t *c2, double *d1)
+{
+    if (mode == SOMHIP_RERANK_FP64)
+        hipLaunchKernelGGL((bmu_exact_wave_kernel<DP, SOMHIP_RERANK_FP64>), dim3((unsigned)blocks), dim3(256), 0, st,
+                           x, dim, list, list_count, w, mn, default_index, c1, c2, d1);
+    else
+        hipLaunchKernelGGL((bmu_exact_wave_kernel<DP, SOMHIP_RERANK_REF32>), dim3((unsigned)blocks), dim3(256), 0, st,
+                           x, dim, list, list_count, w, mn, default_index, c1, c2, d1);
 }
 
 constexpr int G_FP32 = 2;
@@ -764,7 +848,8 @@ BmuModelDev somhip_model_dev(const somhip_model *M)
     d.wmax = M->wmax;
     d.scal = M->scal;
     d.wperm = M->wperm;
-    d.nblk = M->nblk;
+    d.nblk = M->pblk;
+    d.nblk_a = M->pblk_a;
     d.mn = M->mn;
     d.dim = M->dim;
     return d;
@@ -791,7 +876,7 @@ int somhip_launch_bmu_tiles(hipStream_t st, int dpad, int arithmetic, int need_c
     do {                                                                                                  \
         hipLaunchKernelGGL((bmu_bf16_tiles_kernel<D, G_BF16, MO, C>), g, b, 0, st, models, tiles, fb_list, \
                            fb_count);                                                                    \
-        hipLaunchKernelGGL((bmu_exact_tiles_kernel<D, MO, C>), dim3(fb_grid), dim3(256), 0, st, models,    \
+        hipLaunchKernelGGL((bmu_exact_tiles_kernel<D, MO>), dim3(fb_grid), dim3(256), 0, st, models,       \
                            tiles, fb_list, fb_count);                                                    \
     } while (0)
 #define SOM_TL2(D)                                                                                        \
@@ -818,9 +903,9 @@ int somhip_prepare(somhip_model *M)
     hipLaunchKernelGGL(prep_centre_kernel, dim3(M->dpad), dim3(256), 0, st, M->w, M->mn, M->dim, M->dpad, M->centre);
     SOM_HIP_TRY(hipMemsetAsync(M->wmax, 0, sizeof(float) * M->dpad, st));
     SOM_HIP_TRY(hipMemsetAsync(M->scal, 0, sizeof(float) * 4, st));
-    const int slots = M->nblk * 32;
+    const int slots = std::max(M->nblk, M->pblk) * 32;
     hipLaunchKernelGGL(prep_rows_kernel, dim3((slots + 255) / 256), dim3(256), 0, st, M->w, M->mn, M->dim, M->dpad,
-                       M->nblk, M->centre, M->perm, M->frag_bf, M->frag_f32, M->wperm, M->norms_perm, M->norms_nat,
+                       M->nblk, M->pblk, M->centre, M->perm, M->frag_bf, M->frag_f32, M->wperm, M->norms_perm, M->norms_nat,
                        (unsigned int *)M->wmax, (unsigned int *)M->scal);
     SOM_HIP_TRY(hipGetLastError());
     M->prepared = true;
@@ -889,7 +974,7 @@ int somhip_launch_bmu(somhip_model *M, const float *x, int64_t n, int precision,
         somhip_set_error("unknown precision");
         return SOMHIP_ERR_ARG;
     }
-    if (M->nblk * 32 > 65536) {
+    if (M->pblk * 32 > 65536) {
         somhip_set_error("BF16_RERANK supports at most 65536 neurons");
         return SOMHIP_ERR_UNSUPPORTED;
     }
@@ -905,10 +990,11 @@ int somhip_launch_bmu(somhip_model *M, const float *x, int64_t n, int precision,
         SOM_HIP_TRY(hipGetLastError());
     }
     TimerScope tf(M->ctx, TK_BMU_FALLBACK);
-    // Fallback over the flagged samples: a fixed grid that reads the device-side count.
-    const int64_t fb_items = std::min<int64_t>(n, (int64_t)M->ctx->num_cus * 256);
-    if (dp == 16) launch_exact<16>(arithmetic, need_c2, st, x, n, M->dim, fb_list, fb_count, fb_items, M->w, M->mn, default_index, c1, c2, d1, nullptr);
-    else launch_exact<32>(arithmetic, need_c2, st, x, n, M->dim, fb_list, fb_count, fb_items, M->w, M->mn, default_index, c1, c2, d1, nullptr);
+    // Fallback over the flagged samples: one wave per sample, a fixed grid that reads the device-side
+    // count (an empty list costs one launch).
+    const int64_t fb_blocks = std::max<int64_t>(1, std::min<int64_t>((n + 3) / 4, (int64_t)M->ctx->num_cus * 8));
+    if (dp == 16) launch_exact_wave<16>(arithmetic, st, x, M->dim, fb_list, fb_count, fb_blocks, M->w, M->mn, default_index, c1, c2, d1);
+    else launch_exact_wave<32>(arithmetic, st, x, M->dim, fb_list, fb_count, fb_blocks, M->w, M->mn, default_index, c1, c2, d1);
     SOM_HIP_TRY(hipGetLastError());
     return SOMHIP_OK;
 }

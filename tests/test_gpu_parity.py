@@ -89,6 +89,8 @@ CASES = [
     (8, 12, 16, 4099, "hexagonal", 6),
     (16, 16, 17, 2048, "hexagonal", 7),
     (64, 64, 16, 6000, "hexagonal", 8),
+    (32, 32, 16, 9000, "hexagonal", 11),
+    (45, 50, 32, 3000, "square", 12),
     (20, 30, 32, 3000, "square", 9),
     (7, 9, 40, 700, "square", 10),
 ]
@@ -211,6 +213,31 @@ def test_one_step_vs_fp64_factorised(precision):
     Wf, c1, _, _ = R.factorised_epoch_fp64(W0, R.topography(m, n), X, None, sigma)
     rel = np.abs(som.weights - Wf) / np.maximum(np.abs(Wf), 1e-3)
     assert rel.max() <= 1e-6, rel.max()
+
+
+@pytest.mark.parametrize("shape", [(64, 64, 16, "hexagonal"), (32, 40, 32, "square")])
+def test_trained_map_bf16_equals_exact(shape):
+    """On a map with trained structure (near-equal neighbour distances, the hard case for the BF16
+    window): BF16 training equals exact training bit for bit, and the BF16 winners / errors on the
+    trained map equal the oracle's."""
+    from decontamination_amd import synth
+    m, n, dim, topo = shape
+    N = 20000
+    X = synth.catalogue_rows(N, 8, dim - 10, seed=3)
+    W0 = synth.uniform_weights(m * n, dim, seed=4)
+    soms = {}
+    for precision in ("exact", "bf16_rerank"):
+        som = dam.SOM_Batch(m, n, dim, topology=topo, precision=precision)
+        som._weights[...] = W0
+        som.train(X, n_epochs=4, use_best_epoch=False)
+        soms[precision] = som
+    assert np.array_equal(soms["exact"].weights, soms["bf16_rerank"].weights)
+    assert np.array_equal(soms["exact"].quantization_errors, soms["bf16_rerank"].quantization_errors)
+    W = soms["exact"].weights
+    assert np.array_equal(soms["bf16_rerank"].get_winners(X), R.find_bmus(W, X, REF32))
+    q, t = soms["bf16_rerank"].compute_errors(X)
+    eq, et = R.compute_errors(W, R.topography(m, n, topo), X, None, topo, REF32)
+    assert np.isclose(q, eq, rtol=1e-12, atol=0) and np.isclose(t, et, rtol=1e-12, atol=0)
 
 
 def test_determinism_and_chunking():
