@@ -132,20 +132,44 @@ def cpu_baseline(X, W0, topo, sigma0, E, target_s=12.0):
                       f"{t2:.2f} s; extrapolated per sample"}
 
 
-def accuracy(ctx, res, X, topo, m, prec, arith, n_sub=65536):
-    """BMU disagreement of the GPU path against the FP64 oracle on a subsample, at the run's final weights."""
+def accuracy(ctx, res, X, topo, m, prec, arith, E, sigma0, n_sub=1 << 20):
+    """Against the FP64 oracle on a random subsample, at the run's final weights W: BMU disagreement,
+    quantisation error, and the centroids after one more epoch (schedule epoch E-1) on the subsample
+    through the GPU path versus the FP64 factorised epoch on the same rows."""
     import decontamination_amd as dam
+    from decontamination_amd import _lib
+    from decontamination_amd.runtime import Accumulator, DeviceArray, Model
     from oracle import som_ref as R
     rng = np.random.default_rng(123)
     idx = np.sort(rng.choice(len(X), size=min(n_sub, len(X)), replace=False))
     Xs = np.ascontiguousarray(X[idx])
     W = res["W"]
-    som = dam.SOM_Batch(m, m, W.shape[1], precision=prec, arithmetic=arith)
+    mn, dim = W.shape
+    som = dam.SOM_Batch(m, m, dim, precision=prec, arithmetic=arith)
     som._weights[...] = W
     got = som.get_winners(Xs)
-    c1, c2, d1 = R.top2_fp64_np(W, Xs)
+    qe, _ = som.compute_errors(Xs)
+    model = Model(ctx, m, m, dim, "hexagonal", topo)
+    model.set_weights(W)
+    acc = Accumulator(model)
+    xd = DeviceArray.from_host(ctx, Xs)
+    acc.reset()
+    acc.epoch_accumulate(xd.ptr, None, len(Xs), _lib.PRECISIONS[prec], _lib.ARITHMETICS[arith])
+    acc.epoch_update(E - 1, E, sigma0)
+    Wg = model.get_weights().astype(np.float64)
+    acc.close()
+    xd.free()
+    model.close()
+    W64, c1, _, d1 = R.factorised_epoch_fp64(W, topo, Xs, None, R.sigma_at(sigma0, E - 1, E), out_dtype=np.float64)
+    qe64 = float(np.mean(np.sqrt(d1)))
+    err = np.abs(Wg - W64)
+    scale = np.abs(W64).max()
     dis = float(np.mean(got != c1))
-    return {"subsample": len(idx), "bmu_disagreement_vs_fp64": dis, "bmu_agreement_vs_fp64": 1.0 - dis}
+    return {"subsample": len(idx), "bmu_disagreement_vs_fp64": dis, "bmu_agreement_vs_fp64": 1.0 - dis,
+            "qe": qe, "qe_fp64": qe64, "qe_rel_err": abs(qe - qe64) / qe64,
+            "centroid_max_abs_err": float(err.max()),
+            "centroid_max_rel_err": float((err / np.maximum(np.abs(W64), 1e-3 * scale)).max()),
+            "centroid_rel_err_def": "max |W_gpu - W_fp64| / max(|W_fp64|, 1e-3 max|W_fp64|) over all entries"}
 
 
 def serving_bench(X16, seconds, rate, budget_gb, seed=0):
@@ -334,7 +358,8 @@ def main():
     if not args.no_accuracy:
         acc = {}
         for prec, r in results.items():
-            acc[prec] = accuracy(ctx, r, X, topo, m, prec, args.arithmetic)
+            acc[prec] = accuracy(ctx, r, X, topo, m, prec, args.arithmetic, E, sigma0)
+            log(f"accuracy {prec}: {json.dumps(acc[prec])}")
         out["accuracy"] = acc
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(X, W0, topo, sigma0, E)

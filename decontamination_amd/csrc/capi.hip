@@ -393,7 +393,7 @@ int somhip_model_create(somhip_ctx *C, int m, int n, int dim, int topology, cons
     SOM_ALLOC(M->norms_nat, sizeof(float) * slots);
     SOM_ALLOC(M->frag_bf, sizeof(unsigned short) * pslots * M->dpad * 2);
     SOM_ALLOC(M->frag_f32, sizeof(float) * slots * M->dpad);
-    SOM_ALLOC(M->wperm, sizeof(float) * pslots * M->dpad);
+    SOM_ALLOC(M->wslot, sizeof(float) * pslots * M->dpad);
     SOM_ALLOC(M->perm, sizeof(int) * pslots);
 #undef SOM_ALLOC
     if ((e = hipMemcpyAsync(M->perm, layout.perm.data(), sizeof(int) * pslots, hipMemcpyHostToDevice, C->stream)) != hipSuccess)
@@ -412,7 +412,7 @@ int somhip_model_destroy(somhip_model *M)
     if (!M) return SOMHIP_OK;
     if (M->ctx) (void)hipStreamSynchronize(M->ctx->stream);
     void *ptrs[] = {M->w,       M->topo,      M->centre,  M->wmax,     M->scal, M->norms_perm,
-                    M->norms_nat, M->frag_bf, M->frag_f32, M->wperm, M->perm};
+                    M->norms_nat, M->frag_bf, M->frag_f32, M->wslot, M->perm};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     delete M;

@@ -105,7 +105,7 @@ struct BmuModelDev {
     const float *centre;    // weight centroid (dpad)
     const float *wmax;      // max_j |w'_jk| (dpad)
     const float *scal;      // [0] max_j |w'_j|^2
-    const float *wperm;     // FP32 centred weights per permuted slot, (slots, dpad) row-major
+    const float *wslot;     // FP32 weights (as given) per permuted slot, (slots, dpad), zero padded
     int nblk;               // 32-slot blocks of the permuted order
     int nblk_a;             // leading blocks that hold the coarse sub-lattice (== nblk: none)
     int mn, dim;
@@ -221,7 +221,7 @@ struct somhip_model {
     float *norms_nat = nullptr;     // (nblk*32) |w'|^2 in natural order (padding slots = +inf)
     unsigned short *frag_bf = nullptr;  // bf16 A fragments [blk][kc][hi,lo][64 lanes][8]
     float *frag_f32 = nullptr;      // FP32 A fragments [blk][dpad/8][64 lanes][4]
-    float *wperm = nullptr;         // FP32 centred weights per permuted slot [slot][dpad]
+    float *wslot = nullptr;         // FP32 weights (as given) per permuted slot [slot][dpad], zero padded
     int *perm = nullptr;            // (pblk*32) slot -> neuron (or -1 for padding)
 };
 

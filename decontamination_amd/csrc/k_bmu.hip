@@ -336,7 +336,7 @@ __device__ __forceinline__ void bmu_bf16_body(const BmuModelDev &Md, const float
     const uint4 *__restrict__ gfrag = (const uint4 *)Md.frag;
     const uint4 *__restrict__ gnorm = (const uint4 *)Md.norms4;
     const int *__restrict__ perm = Md.perm;
-    const float *__restrict__ w = Md.w;
+    const float *__restrict__ wslot = Md.wslot;
     const int nblk = Md.nblk, dim = Md.dim;
     const int tid = threadIdx.x;
     const int lane = tid & (WAVE - 1);
@@ -558,14 +558,28 @@ __device__ __forceinline__ void bmu_bf16_body(const BmuModelDev &Md, const float
             float xr[DP];
 #pragma unroll
             for (int k = 0; k < DP; k++) xr[k] = k < dim ? x[s * dim + k] : 0.f;
-            for (int e = 0; e < mycnt; e++) {
+            // candidate rows come from the slot-ordered copy of W (no index indirection before the
+            // load); the next candidate's row is loaded while this one is evaluated
+            auto load_row = [&](int e, float (&wr)[DP], int &j) {
                 const int slot = cw[g][e][lane];
-                const int j = perm[slot];
-                if (j < 0) continue;
+                j = perm[slot];
+                const f32x4 *row = (const f32x4 *)(wslot + (int64_t)slot * DP);
+#pragma unroll
+                for (int q = 0; q < DP / 4; q++) {
+                    const f32x4 t = row[q];
+                    wr[4 * q] = t[0]; wr[4 * q + 1] = t[1]; wr[4 * q + 2] = t[2]; wr[4 * q + 3] = t[3];
+                }
+            };
+            float wn[DP];
+            int jn = -1;
+            if (mycnt > 0) load_row(0, wn, jn);
+            for (int e = 0; e < mycnt; e++) {
                 float wr[DP];
 #pragma unroll
-                for (int k = 0; k < DP; k++) wr[k] = k < dim ? w[(int64_t)j * dim + k] : 0.f;
-                tp.push(sqd_mode<MODE>(wr, xr, DP), j);
+                for (int k = 0; k < DP; k++) wr[k] = wn[k];
+                const int j = jn;
+                if (e + 1 < mycnt) load_row(e + 1, wn, jn);
+                if (j >= 0) tp.push(sqd_mode<MODE>(wr, xr, DP), j);  // zero-padded dims add exact zeros
             }
         }
         // merge with the partner lane
@@ -715,7 +729,7 @@ __global__ void prep_centre_kernel(const float *__restrict__ w, int mn, int dim,
 __global__ void prep_rows_kernel(const float *__restrict__ w, int mn, int dim, int dpad, int nblk, int pblk,
                                  const float *__restrict__ centre, const int *__restrict__ perm,
                                  unsigned short *__restrict__ frag_bf, float *__restrict__ frag_f32,
-                                 float *__restrict__ wperm, float *__restrict__ norms_perm,
+                                 float *__restrict__ wslot, float *__restrict__ norms_perm,
                                  float *__restrict__ norms_nat,
                                  unsigned int *__restrict__ wmax_bits, unsigned int *__restrict__ nwmax_bits)
 {
@@ -734,7 +748,7 @@ __global__ void prep_rows_kernel(const float *__restrict__ w, int mn, int dim, i
             const unsigned short l = f32_to_bf16_bits(__fsub_rn(v, bf16_bits_to_f32(h)));
             const int kc = d >> 4, half = (d & 15) >> 3, e = d & 7;
             const int lane = row + 32 * half;
-            wperm[(int64_t)p * dpad + d] = v;
+            wslot[(int64_t)p * dpad + d] = (j >= 0 && d < dim) ? w[(int64_t)j * dim + d] : 0.f;
             frag_bf[((((int64_t)blk * kcn + kc) * 2 + 0) * 64 + lane) * 8 + e] = h;
             frag_bf[((((int64_t)blk * kcn + kc) * 2 + 1) * 64 + lane) * 8 + e] = l;
             if (j >= 0) atomicMax(&wmax_bits[d], __builtin_bit_cast(unsigned int, fabsf(v)));
@@ -847,7 +861,7 @@ BmuModelDev somhip_model_dev(const somhip_model *M)
     d.centre = M->centre;
     d.wmax = M->wmax;
     d.scal = M->scal;
-    d.wperm = M->wperm;
+    d.wslot = M->wslot;
     d.nblk = M->pblk;
     d.nblk_a = M->pblk_a;
     d.mn = M->mn;
@@ -905,7 +919,7 @@ int somhip_prepare(somhip_model *M)
     SOM_HIP_TRY(hipMemsetAsync(M->scal, 0, sizeof(float) * 4, st));
     const int slots = std::max(M->nblk, M->pblk) * 32;
     hipLaunchKernelGGL(prep_rows_kernel, dim3((slots + 255) / 256), dim3(256), 0, st, M->w, M->mn, M->dim, M->dpad,
-                       M->nblk, M->pblk, M->centre, M->perm, M->frag_bf, M->frag_f32, M->wperm, M->norms_perm, M->norms_nat,
+                       M->nblk, M->pblk, M->centre, M->perm, M->frag_bf, M->frag_f32, M->wslot, M->norms_perm, M->norms_nat,
                        (unsigned int *)M->wmax, (unsigned int *)M->scal);
     SOM_HIP_TRY(hipGetLastError());
     M->prepared = true;

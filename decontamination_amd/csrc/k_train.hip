@@ -13,7 +13,9 @@
 
 namespace {
 
-constexpr size_t LDS_SCATTER_MAX = 160 * 1024;
+constexpr size_t LDS_SCATTER_MAX = 160 * 1024;  // neuron slice per scatter workgroup (all of gfx950's LDS)
+constexpr int SCATTER_U = 4;                    // samples per thread and step
+constexpr int SCATTER_NT = 1024;                // threads per scatter workgroup
 
 // ------------------------------------------------------------------------------------------------
 // Column-wise max |v| over finite values; out[k] for k < dim is max|x_k|, out[dim] is max rho (as
@@ -144,39 +146,43 @@ __global__ __launch_bounds__(1024) void shifts_parts_kernel(const float *__restr
 //   nv = dim + 1 columns (x != NULL: rho*x_0..rho*x_{dim-1}, rho) or nv = 1 (x == NULL: rho).
 //   Samples with bmu < 0 or !(rho > 0) are skipped. rho_f32: round rho to float first (training).
 // ------------------------------------------------------------------------------------------------
-template <int DP>
-__global__ __launch_bounds__(512) void scatter_fixed_kernel(const int32_t *__restrict__ bmu,
-                                                            const float *__restrict__ x, int dim,
-                                                            const double *__restrict__ rho, int rho_f32, int64_t n,
-                                                            int64_t per_wg, const int *__restrict__ shifts, int mn,
-                                                            int nv, int slice_len,
-                                                            unsigned long long *__restrict__ partial)
+// Workgroup r adds the samples of its contiguous range into LDS, one neuron slice at a time (the
+// slice fills the LDS), and writes each slice to partial[r]. The BMU indices of the next U*NT samples
+// are loaded while the current ones are added.
+template <int DP, int U, int NT>
+__global__ __launch_bounds__(NT) void scatter_fixed_kernel(const int32_t *__restrict__ bmu,
+                                                           const float *__restrict__ x, int dim,
+                                                           const double *__restrict__ rho, int rho_f32, int64_t n,
+                                                           int64_t per_wg, const int *__restrict__ shifts, int mn,
+                                                           int nv, int slice_len,
+                                                           unsigned long long *__restrict__ partial)
 {
-    constexpr int U = 4;  // samples in flight per thread
     extern __shared__ unsigned long long lds[];
     const int64_t i0 = (int64_t)blockIdx.x * per_wg;
     const int64_t i1 = min(n, i0 + per_wg);
     unsigned long long *out = partial + (int64_t)blockIdx.x * mn * nv;
-    double scale[DP];
+    auto load_bmu = [&](int64_t ib, int (&c)[U]) {
 #pragma unroll
-    for (int k = 0; k < DP; k++) scale[k] = (k < nv - 1) ? ldexp(1.0, shifts[k]) : 0.0;
-    const double srho = ldexp(1.0, shifts[nv - 1]);
+        for (int u = 0; u < U; u++) {
+            const int64_t i = ib + (int64_t)u * NT;
+            c[u] = i < i1 ? bmu[i] : -1;
+        }
+    };
     for (int c0 = 0; c0 < mn; c0 += slice_len) {
         const int cl = min(slice_len, mn - c0);
-        for (int e = threadIdx.x; e < cl * nv; e += blockDim.x) lds[e] = 0ull;
+        for (int e = threadIdx.x; e < cl * nv; e += NT) lds[e] = 0ull;
         __syncthreads();
-        for (int64_t ib = i0 + threadIdx.x; ib < i1; ib += (int64_t)U * blockDim.x) {
+        int cn[U];
+        load_bmu(i0 + threadIdx.x, cn);
+        for (int64_t ib = i0 + threadIdx.x; ib < i1; ib += (int64_t)U * NT) {
             int cc[U];
 #pragma unroll
-            for (int u = 0; u < U; u++) {
-                const int64_t i = ib + (int64_t)u * blockDim.x;
-                const int c = i < i1 ? bmu[i] : -1;
-                cc[u] = (c >= c0 && c < c0 + cl) ? c - c0 : -1;
-            }
+            for (int u = 0; u < U; u++) cc[u] = (cn[u] >= c0 && cn[u] < c0 + cl) ? cn[u] - c0 : -1;
+            load_bmu(ib + (int64_t)U * NT, cn);
 #pragma unroll
             for (int u = 0; u < U; u++) {
                 if (cc[u] < 0) continue;
-                const int64_t i = ib + (int64_t)u * blockDim.x;
+                const int64_t i = ib + (int64_t)u * NT;
                 double r = rho ? rho[i] : 1.0;
                 if (rho_f32) r = (double)(float)r;
                 if (!(r > 0.0) || !(r < 1e300)) continue;
@@ -187,18 +193,16 @@ __global__ __launch_bounds__(512) void scatter_fixed_kernel(const int32_t *__res
                     for (int k = 0; k < DP; k++) xv[k] = k < dim ? x[i * dim + k] : 0.f;
 #pragma unroll
                     for (int k = 0; k < DP; k++) {
-                        if (k < dim) {
-                            const long long q = llrint(r * (double)xv[k] * scale[k]);
-                            atomicAdd(&row[k], (unsigned long long)q);
-                        }
+                        // r * x is exact in FP64 (24-bit mantissas), and so is the power-of-two scaling
+                        if (k < dim)
+                            atomicAdd(&row[k], (unsigned long long)llrint(ldexp(r * (double)xv[k], shifts[k])));
                     }
                 }
-                const long long qr = llrint(r * srho);
-                atomicAdd(&row[nv - 1], (unsigned long long)qr);
+                atomicAdd(&row[nv - 1], (unsigned long long)llrint(ldexp(r, shifts[nv - 1])));
             }
         }
         __syncthreads();
-        for (int e = threadIdx.x; e < cl * nv; e += blockDim.x) out[(int64_t)c0 * nv + e] = lds[e];
+        for (int e = threadIdx.x; e < cl * nv; e += NT) out[(int64_t)c0 * nv + e] = lds[e];
         __syncthreads();
     }
 }
@@ -467,10 +471,10 @@ int somhip_launch_scatter(hipStream_t st, const int32_t *bmu, const float *x, in
     const int dpad = (dim + 15) / 16 * 16;
 #define SOM_SC(D)                                                                                              \
     do {                                                                                                       \
-        SOM_HIP_TRY(hipFuncSetAttribute((const void *)scatter_fixed_kernel<D>,                                 \
-                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));                \
-        hipLaunchKernelGGL(scatter_fixed_kernel<D>, dim3(nwg), dim3(512), lds, st, bmu, x, dim, rho, rho_f32, n, \
-                           per_wg, shifts, mn, nv, slice_len, partial);                                        \
+        auto kern = scatter_fixed_kernel<D, SCATTER_U, SCATTER_NT>;                                            \
+        SOM_HIP_TRY(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds)); \
+        hipLaunchKernelGGL(kern, dim3(nwg), dim3(SCATTER_NT), lds, st, bmu, x, dim, rho, rho_f32, n, per_wg,     \
+                           shifts, mn, nv, slice_len, partial);                                                \
     } while (0)
     if (!x || dpad <= 16) SOM_SC(16);
     else if (dpad == 32) SOM_SC(32);

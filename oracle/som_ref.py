@@ -206,19 +206,19 @@ def sigma_at(sigma0, e, etot):
     return max(s, FLOAT_EPS)
 
 
-def factorised_epoch_fp64(W, topo, X, rho, sigma, bmu=None):
+def factorised_epoch_fp64(W, topo, X, rho, sigma, bmu=None, out_dtype=np.float32):
     """One FP64 batch epoch in the factorised form num = H.S, den = H.c (algebraically identical to
     som_batch.py:509-566 in epoch mode because h depends only on (j, bmu(i))). Returns
-    (W_new float32, bmu, c2, d1)."""
+    (W_new in out_dtype, bmu, c2, d1)."""
     mn, dim = np.asarray(W).shape
     c1, c2, d1 = top2_fp64_np(W, X) if bmu is None else (bmu, None, None)
     r = np.ones(X.shape[0]) if rho is None else np.asarray(rho, np.float64)
     keep = r > 0
-    S = np.zeros((mn, dim)); cnt = np.zeros(mn)
-    np.add.at(S, c1[keep], np.asarray(X, np.float32)[keep].astype(np.float64) * r[keep, None])
-    np.add.at(cnt, c1[keep], r[keep])
+    xk = np.asarray(X, np.float32)[keep].astype(np.float64)
+    S = np.stack([np.bincount(c1[keep], weights=xk[:, k] * r[keep], minlength=mn) for k in range(dim)], 1)
+    cnt = np.bincount(c1[keep], weights=r[keep], minlength=mn)
     H = neighbourhood_fp64(topo, sigma)
     num = H @ S
     den = H @ cnt
     Wn = np.where(den[:, None] > FLOAT_EPS, num / np.where(den[:, None] > FLOAT_EPS, den[:, None], 1.0), 0.0)
-    return Wn.astype(np.float32), c1, c2, d1
+    return Wn.astype(out_dtype), c1, c2, d1
