@@ -14,7 +14,6 @@
 namespace {
 
 constexpr size_t LDS_SCATTER_MAX = 160 * 1024;  // neuron slice per scatter workgroup (all of gfx950's LDS)
-constexpr int SCATTER_U = 4;                    // samples per thread and step
 constexpr int SCATTER_NT = 1024;                // threads per scatter workgroup
 
 // ------------------------------------------------------------------------------------------------
@@ -147,50 +146,57 @@ __global__ __launch_bounds__(1024) void shifts_parts_kernel(const float *__restr
 //   Samples with bmu < 0 or !(rho > 0) are skipped. rho_f32: round rho to float first (training).
 // ------------------------------------------------------------------------------------------------
 // Workgroup r adds the samples of its contiguous range into LDS, one neuron slice at a time (the
-// slice fills the LDS), and writes each slice to partial[r]. The BMU indices of the next U*NT samples
-// are loaded while the current ones are added.
-template <int DP, int U, int NT>
-__global__ __launch_bounds__(NT) void scatter_fixed_kernel(const int32_t *__restrict__ bmu,
-                                                           const float *__restrict__ x, int dim,
-                                                           const double *__restrict__ rho, int rho_f32, int64_t n,
-                                                           int64_t per_wg, const int *__restrict__ shifts, int mn,
-                                                           int nv, int slice_len,
-                                                           unsigned long long *__restrict__ partial)
+// slice fills the LDS), and writes each slice to partial[r]. Per slice, every wave sweeps its samples'
+// BMU indices (coalesced), queues the ones inside the slice in a per-wave LDS ring, and whenever 64
+// are queued adds them with all lanes busy: one sample per lane, its row loaded as 16-byte vectors
+// when XVEC (dim % 4 == 0, 16-byte aligned rows), then one 64-bit LDS atomic per column.
+template <int DP, bool XVEC>
+__global__ __launch_bounds__(SCATTER_NT) void scatter_fixed_kernel(const int32_t *__restrict__ bmu,
+                                                                   const float *__restrict__ x, int dim,
+                                                                   const double *__restrict__ rho, int rho_f32,
+                                                                   int64_t n, int64_t per_wg,
+                                                                   const int *__restrict__ shifts, int mn, int nv,
+                                                                   int slice_len,
+                                                                   unsigned long long *__restrict__ partial)
 {
+    constexpr int W = 64;
+    constexpr int NWV = SCATTER_NT / W;
+    constexpr int QCAP = 2 * W;  // ring entries per wave
     extern __shared__ unsigned long long lds[];
+    unsigned int *q_idx = (unsigned int *)(lds + (size_t)slice_len * nv);   // [NWV][QCAP] sample - i0
+    unsigned short *q_slot = (unsigned short *)(q_idx + NWV * QCAP);       // [NWV][QCAP] neuron - c0
+    const int lane = threadIdx.x & (W - 1);
+    const int wave = threadIdx.x / W;
+    unsigned int *qi = q_idx + wave * QCAP;
+    unsigned short *qs = q_slot + wave * QCAP;
     const int64_t i0 = (int64_t)blockIdx.x * per_wg;
     const int64_t i1 = min(n, i0 + per_wg);
     unsigned long long *out = partial + (int64_t)blockIdx.x * mn * nv;
-    auto load_bmu = [&](int64_t ib, int (&c)[U]) {
-#pragma unroll
-        for (int u = 0; u < U; u++) {
-            const int64_t i = ib + (int64_t)u * NT;
-            c[u] = i < i1 ? bmu[i] : -1;
-        }
-    };
-    for (int c0 = 0; c0 < mn; c0 += slice_len) {
-        const int cl = min(slice_len, mn - c0);
-        for (int e = threadIdx.x; e < cl * nv; e += NT) lds[e] = 0ull;
-        __syncthreads();
-        int cn[U];
-        load_bmu(i0 + threadIdx.x, cn);
-        for (int64_t ib = i0 + threadIdx.x; ib < i1; ib += (int64_t)U * NT) {
-            int cc[U];
-#pragma unroll
-            for (int u = 0; u < U; u++) cc[u] = (cn[u] >= c0 && cn[u] < c0 + cl) ? cn[u] - c0 : -1;
-            load_bmu(ib + (int64_t)U * NT, cn);
-#pragma unroll
-            for (int u = 0; u < U; u++) {
-                if (cc[u] < 0) continue;
-                const int64_t i = ib + (int64_t)u * NT;
-                double r = rho ? rho[i] : 1.0;
-                if (rho_f32) r = (double)(float)r;
-                if (!(r > 0.0) || !(r < 1e300)) continue;
-                unsigned long long *row = lds + (int64_t)cc[u] * nv;
+
+    // adds queue entries [h, h + cnt) (cnt <= 64), one per lane
+    auto drain = [&](unsigned h, unsigned cnt) {
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        if ((unsigned)lane < cnt) {
+            const unsigned e = (h + lane) & (QCAP - 1);
+            const int64_t i = i0 + qi[e];
+            unsigned long long *row = lds + (int64_t)qs[e] * nv;
+            double r = rho ? rho[i] : 1.0;
+            if (rho_f32) r = (double)(float)r;
+            if (r > 0.0 && r < 1e300) {
                 if (x) {
                     float xv[DP];
+                    if (XVEC) {
+                        const f32x4 *xr = (const f32x4 *)(x + i * dim);
 #pragma unroll
-                    for (int k = 0; k < DP; k++) xv[k] = k < dim ? x[i * dim + k] : 0.f;
+                        for (int v = 0; v < DP / 4; v++) {
+                            const f32x4 t = 4 * v < dim ? xr[v] : f32x4{0.f, 0.f, 0.f, 0.f};
+                            xv[4 * v] = t[0]; xv[4 * v + 1] = t[1]; xv[4 * v + 2] = t[2]; xv[4 * v + 3] = t[3];
+                        }
+                    } else {
+#pragma unroll
+                        for (int k = 0; k < DP; k++) xv[k] = k < dim ? x[i * dim + k] : 0.f;
+                    }
 #pragma unroll
                     for (int k = 0; k < DP; k++) {
                         // r * x is exact in FP64 (24-bit mantissas), and so is the power-of-two scaling
@@ -201,8 +207,41 @@ __global__ __launch_bounds__(NT) void scatter_fixed_kernel(const int32_t *__rest
                 atomicAdd(&row[nv - 1], (unsigned long long)llrint(ldexp(r, shifts[nv - 1])));
             }
         }
+        __builtin_amdgcn_wave_barrier();
+    };
+
+    for (int c0 = 0; c0 < mn; c0 += slice_len) {
+        const int cl = min(slice_len, mn - c0);
+        for (int e = threadIdx.x; e < cl * nv; e += SCATTER_NT) lds[e] = 0ull;
         __syncthreads();
-        for (int e = threadIdx.x; e < cl * nv; e += NT) out[(int64_t)c0 * nv + e] = lds[e];
+        unsigned head = 0, tail = 0;  // wave-uniform ring positions
+        int64_t ib = i0 + (int64_t)wave * W + lane;
+        int cn = ib < i1 ? bmu[ib] : -1;
+        for (; ib - lane < i1; ib += SCATTER_NT) {
+            const int c = cn;
+            const int64_t inext = ib + SCATTER_NT;
+            cn = inext < i1 ? bmu[inext] : -1;
+            const bool in = c >= c0 && c < c0 + cl;
+            const unsigned long long m = __builtin_amdgcn_ballot_w64(in);
+            const unsigned below = __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+            if (in) {
+                const unsigned e = (tail + below) & (QCAP - 1);
+                qi[e] = (unsigned)(ib - i0);
+                qs[e] = (unsigned short)(c - c0);
+            }
+            tail += (unsigned)__builtin_popcountll(m);
+            if (tail - head >= (unsigned)W) {
+                drain(head, W);
+                head += W;
+            }
+        }
+        while (tail != head) {
+            const unsigned cnt = min(tail - head, (unsigned)W);
+            drain(head, cnt);
+            head += cnt;
+        }
+        __syncthreads();
+        for (int e = threadIdx.x; e < cl * nv; e += SCATTER_NT) out[(int64_t)c0 * nv + e] = lds[e];
         __syncthreads();
     }
 }
@@ -459,19 +498,25 @@ int somhip_launch_scatter(hipStream_t st, const int32_t *bmu, const float *x, in
                           double *acc, size_t lds_limit)
 {
     if (n <= 0) return SOMHIP_OK;
-    const size_t lds_bytes = std::min(LDS_SCATTER_MAX, lds_limit);
-    int slice_len = (int)(lds_bytes / ((size_t)nv * 8));
+    constexpr size_t queue_bytes = (size_t)(SCATTER_NT / 64) * 128 * (4 + 2);
+    const size_t lds_bytes = std::min(LDS_SCATTER_MAX, lds_limit) - queue_bytes;
+    int slice_len = std::min((int)(lds_bytes / ((size_t)nv * 8)), 65536);
     if (slice_len < 1) {
         somhip_set_error("LDS too small for one neuron row");
         return SOMHIP_ERR_UNSUPPORTED;
     }
     if (slice_len > mn) slice_len = mn;
-    const size_t lds = (size_t)slice_len * nv * 8;
+    const size_t lds = (size_t)slice_len * nv * 8 + queue_bytes;
     const int64_t per_wg = (n + nwg - 1) / nwg;
+    if (per_wg > 0xffffffffll) {
+        somhip_set_error("too many samples per scatter workgroup");
+        return SOMHIP_ERR_UNSUPPORTED;
+    }
     const int dpad = (dim + 15) / 16 * 16;
+    const bool xvec = x && dim % 4 == 0 && ((uintptr_t)x & 15) == 0;
 #define SOM_SC(D)                                                                                              \
     do {                                                                                                       \
-        auto kern = scatter_fixed_kernel<D, SCATTER_U, SCATTER_NT>;                                            \
+        auto kern = xvec ? scatter_fixed_kernel<D, true> : scatter_fixed_kernel<D, false>;                     \
         SOM_HIP_TRY(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds)); \
         hipLaunchKernelGGL(kern, dim3(nwg), dim3(SCATTER_NT), lds, st, bmu, x, dim, rho, rho_f32, n, per_wg,     \
                            shifts, mn, nv, slice_len, partial);                                                \
