@@ -506,8 +506,8 @@ __device__ __forceinline__ void bmu_bf16_body(const BmuModelDev &Md, const float
             for (int r = 0; r < 16; r++) bm[r] = __builtin_amdgcn_ballot_w64(v[r] <= T[g]);
 #pragma unroll
             for (int r = 0; r < 16; r++) {
-                if (bm[r] != 0ull) {  // uniform; lanes without a hit write to the spare row
-                    const bool hit = v[r] <= T[g];
+                if (__builtin_expect(bm[r] != 0ull, 0)) {  // uniform, rare: out of line
+                    const bool hit = v[r] <= T[g];  // lanes without a hit write to the spare row
                     cw[g][hit ? min(cnt[g], CAPL) : CAPL][lane] = (unsigned short)(sbase_blk + 8 * (r >> 2) + (r & 3));
                     cnt[g] += hit ? 1 : 0;
                 }
