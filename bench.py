@@ -44,6 +44,8 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--precisions", default="fp32,bf16_rerank")
+    ap.add_argument("--no-fp64-rerank", action="store_true",
+                    help="skip the extra bf16_rerank run with arithmetic='fp64'")
     ap.add_argument("--serve-seconds", type=float, default=20.0)
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--serve-budget-gb", type=float, default=64.0)
@@ -87,14 +89,30 @@ def run_epochs(ctx, model_cls, acc_cls, dev, x_dev, n, W0, topo, m, prec, arith,
     return {"seconds": t1 - t0, "timing": timing, "errors": e3, "W": W, "model": model}
 
 
-def roofline(timing, kernel, flops_per_launch, peak_tflops):
+TRAFFIC_PROFILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                               "som_epoch_64x64_d16_hbm_traffic.md")
+
+
+def measured_traffic(kernel_symbol):
+    """HBM bytes per launch of `kernel_symbol` from the committed FETCH_SIZE/WRITE_SIZE summary (two
+    rocprofv3 --pmc passes of this script, scaled by a device copy of known size), or None."""
+    try:
+        for line in open(TRAFFIC_PROFILE):
+            if line.startswith(f"traffic_bytes_per_launch {kernel_symbol}:"):
+                return int(float(line.split(":")[1]))
+    except OSError:
+        pass
+    return None
+
+
+def roofline(timing, kernel, flops_per_launch, peak_tflops, traffic=None):
     if kernel not in timing:
         return None
     ms, launches = timing[kernel]
     avg_ms = ms / launches
     achieved = flops_per_launch / (avg_ms * 1e-3) / 1e12
     return {"bound": "mfma", "achieved": round(achieved, 3), "peak": peak_tflops, "unit": "TFLOP/s",
-            "frac": round(achieved / peak_tflops, 5), "traffic": None, "kernel": kernel,
+            "frac": round(achieved / peak_tflops, 5), "traffic": traffic, "kernel": kernel,
             "avg_ms": round(avg_ms, 4), "launches": launches}
 
 
@@ -168,8 +186,10 @@ def accuracy(ctx, res, X, topo, m, prec, arith, E, sigma0, n_sub=1 << 20):
     return {"subsample": len(idx), "bmu_disagreement_vs_fp64": dis, "bmu_agreement_vs_fp64": 1.0 - dis,
             "qe": qe, "qe_fp64": qe64, "qe_rel_err": abs(qe - qe64) / qe64,
             "centroid_max_abs_err": float(err.max()),
+            "centroid_rel_err_maxnorm": float(err.max() / scale),
             "centroid_max_rel_err": float((err / np.maximum(np.abs(W64), 1e-3 * scale)).max()),
-            "centroid_rel_err_def": "max |W_gpu - W_fp64| / max(|W_fp64|, 1e-3 max|W_fp64|) over all entries"}
+            "centroid_rel_err_def": "maxnorm: max|W_gpu - W_fp64| / max|W_fp64|; max_rel: max over entries of "
+                                    "|W_gpu - W_fp64| / max(|W_fp64|, 1e-3 max|W_fp64|)"}
 
 
 def serving_bench(X16, seconds, rate, budget_gb, seed=0):
@@ -324,8 +344,18 @@ def main():
         log(f"{prec}: {r['ms_per_step']:.3f} ms/epoch, {r['samples_per_s'] / 1e9:.3f} G samples/s; "
             f"kernels {json.dumps({k: [round(v[0] / v[1], 4), v[1]] for k, v in r['timing'].items()})}")
         results[prec] = r
+    extra = None
+    if "bf16_rerank" in results and args.arithmetic == "ref32" and not args.no_fp64_rerank:
+        # the same path with the rerank in FP64 (arithmetic='fp64'): BMUs of the FP64 formula
+        log(f"timing bf16_rerank with fp64 rerank: warmup {args.warmup}, steps {args.steps}")
+        extra = run_epochs(ctx, Model, Accumulator, DeviceArray, x_dev, N, W0, topo, m, "bf16_rerank", "fp64",
+                           args.steps, args.warmup, E, sigma0)
+        extra["samples_per_s"] = N * args.steps / extra["seconds"]
+        extra["ms_per_step"] = 1e3 * extra["seconds"] / args.steps
+        log(f"bf16_rerank/fp64: {extra['ms_per_step']:.3f} ms/epoch")
 
     flops = 2.0 * N * m * m * dim
+    default_workload = m == 64 and args.nside == 1024
     head = results.get("bf16_rerank") or next(iter(results.values()))
     head_prec = "bf16_rerank" if "bf16_rerank" in results else next(iter(results))
     out = {
@@ -346,7 +376,9 @@ def main():
                                f"{head_prec}, rerank arithmetic {args.arithmetic}",
                    "map": f"{m}x{m}", "dim": dim, "samples": N, "precision": head_prec},
         "roofline": roofline(head["timing"], "bmu_bf16" if head_prec == "bf16_rerank" else "bmu_fp32", flops,
-                             PEAK_BF16_TFLOPS if head_prec == "bf16_rerank" else PEAK_FP32_TFLOPS),
+                             PEAK_BF16_TFLOPS if head_prec == "bf16_rerank" else PEAK_FP32_TFLOPS,
+                             measured_traffic("bmu_bf16_kernel") if head_prec == "bf16_rerank" and default_workload
+                             else None),
         "h2d_upload_s": round(h2d_s, 4),
     }
     if "fp32" in results and "bf16_rerank" in results:
@@ -361,6 +393,13 @@ def main():
             acc[prec] = accuracy(ctx, r, X, topo, m, prec, args.arithmetic, E, sigma0)
             log(f"accuracy {prec}: {json.dumps(acc[prec])}")
         out["accuracy"] = acc
+    if extra is not None:
+        out["bf16_rerank_fp64_arithmetic"] = {"value": round(extra["samples_per_s"], 1),
+                                              "ms_per_step": round(extra["ms_per_step"], 4)}
+        if not args.no_accuracy:
+            a64 = accuracy(ctx, extra, X, topo, m, "bf16_rerank", "fp64", E, sigma0)
+            log(f"accuracy bf16_rerank/fp64: {json.dumps(a64)}")
+            out["bf16_rerank_fp64_arithmetic"]["accuracy"] = a64
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(X, W0, topo, sigma0, E)
     out["serving"] = None
