@@ -709,10 +709,14 @@ __global__ __launch_bounds__(256) void bmu_exact_tiles_kernel(const BmuModelDev 
 // ------------------------------------------------------------------------------------------------
 // Preparation of the derived weight data.
 // ------------------------------------------------------------------------------------------------
-__global__ void prep_centre_kernel(const float *__restrict__ w, int mn, int dim, int dpad, float *__restrict__ centre)
+__global__ void prep_centre_kernel(const float *__restrict__ w, int mn, int dim, int dpad, float *__restrict__ centre,
+                                   float *__restrict__ wmax, float *__restrict__ scal)
 {
-    // one block per padded dimension; fixed-order FP64 sum
+    // one block per padded dimension; fixed-order FP64 sum. Also clears the maxima that
+    // prep_rows_kernel raises (same stream, next launch).
     const int k = blockIdx.x;
+    if (threadIdx.x == 0) wmax[k] = 0.f;
+    if (k == 0 && threadIdx.x < 4) scal[threadIdx.x] = 0.f;
     __shared__ double red[256];
     double acc = 0.0;
     if (k < dim)
@@ -726,47 +730,96 @@ __global__ void prep_centre_kernel(const float *__restrict__ w, int mn, int dim,
     if (threadIdx.x == 0) centre[k] = k < dim ? (float)(red[0] / (double)mn) : 0.f;
 }
 
-__global__ void prep_rows_kernel(const float *__restrict__ w, int mn, int dim, int dpad, int nblk, int pblk,
-                                 const float *__restrict__ centre, const int *__restrict__ perm,
-                                 unsigned short *__restrict__ frag_bf, float *__restrict__ frag_f32,
-                                 float *__restrict__ wslot, float *__restrict__ norms_perm,
-                                 float *__restrict__ norms_nat,
-                                 unsigned int *__restrict__ wmax_bits, unsigned int *__restrict__ nwmax_bits)
+// One thread per slot: centred row, |w'|^2 (FP32 fma in dimension order), the bf16 hi/lo fragments
+// and the raw row of the permuted order, the FP32 fragments of the natural order; 16-byte stores.
+// The per-dimension max |w'| and the max |w'|^2 are reduced over the wave before one atomic each.
+template <int DP>
+__global__ __launch_bounds__(256) void prep_rows_kernel(const float *__restrict__ w, int mn, int dim, int nblk,
+                                                        int pblk, const float *__restrict__ centre,
+                                                        const int *__restrict__ perm, uint4 *__restrict__ frag_bf,
+                                                        f32x4 *__restrict__ frag_f32, f32x4 *__restrict__ wslot,
+                                                        float *__restrict__ norms_perm, float *__restrict__ norms_nat,
+                                                        unsigned int *__restrict__ wmax_bits,
+                                                        unsigned int *__restrict__ nwmax_bits)
 {
+    constexpr int KC = DP / 16;
     const int p = blockIdx.x * blockDim.x + threadIdx.x;  // slot (permuted) == neuron slot (natural)
-    const int kcn = dpad / 16;
-    const int ks4 = dpad / 8;
-    // permuted slot -> bf16 fragments and permuted norms
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int blk = p >> 5, row = p & 31;
+    float am[DP];
+#pragma unroll
+    for (int d = 0; d < DP; d++) am[d] = 0.f;
+    float nmax = 0.f;
+    // permuted slot -> bf16 fragments, raw row and permuted norms
     if (p < pblk * 32) {
         const int j = perm[p];
+        float raw[DP], v[DP];
         float nrm = 0.f;
-        const int blk = p >> 5, row = p & 31;
-        for (int d = 0; d < dpad; d++) {
-            float v = (j >= 0 && d < dim) ? __fsub_rn(w[(int64_t)j * dim + d], centre[d]) : 0.f;
-            nrm = __fmaf_rn(v, v, nrm);
-            const unsigned short h = f32_to_bf16_bits(v);
-            const unsigned short l = f32_to_bf16_bits(__fsub_rn(v, bf16_bits_to_f32(h)));
-            const int kc = d >> 4, half = (d & 15) >> 3, e = d & 7;
-            const int lane = row + 32 * half;
-            wslot[(int64_t)p * dpad + d] = (j >= 0 && d < dim) ? w[(int64_t)j * dim + d] : 0.f;
-            frag_bf[((((int64_t)blk * kcn + kc) * 2 + 0) * 64 + lane) * 8 + e] = h;
-            frag_bf[((((int64_t)blk * kcn + kc) * 2 + 1) * 64 + lane) * 8 + e] = l;
-            if (j >= 0) atomicMax(&wmax_bits[d], __builtin_bit_cast(unsigned int, fabsf(v)));
+#pragma unroll
+        for (int d = 0; d < DP; d++) {
+            raw[d] = (j >= 0 && d < dim) ? w[(int64_t)j * dim + d] : 0.f;
+            v[d] = (j >= 0 && d < dim) ? __fsub_rn(raw[d], centre[d]) : 0.f;
+            nrm = __fmaf_rn(v[d], v[d], nrm);
         }
+#pragma unroll
+        for (int q = 0; q < DP / 4; q++)
+            wslot[(int64_t)p * (DP / 4) + q] = f32x4{raw[4 * q], raw[4 * q + 1], raw[4 * q + 2], raw[4 * q + 3]};
+#pragma unroll
+        for (int kc = 0; kc < KC; kc++)
+#pragma unroll
+            for (int half = 0; half < 2; half++) {
+                unsigned int hw[4], lw[4];
+#pragma unroll
+                for (int e = 0; e < 8; e += 2) {
+                    unsigned int hp = 0, lp = 0;
+#pragma unroll
+                    for (int s = 0; s < 2; s++) {
+                        const float x = v[kc * 16 + half * 8 + e + s];
+                        const unsigned short h = f32_to_bf16_bits(x);
+                        const unsigned short l = f32_to_bf16_bits(__fsub_rn(x, bf16_bits_to_f32(h)));
+                        hp |= (unsigned int)h << (16 * s);
+                        lp |= (unsigned int)l << (16 * s);
+                    }
+                    hw[e / 2] = hp;
+                    lw[e / 2] = lp;
+                }
+                const int64_t u = (((int64_t)blk * KC + kc) * 2) * 64 + row + 32 * half;
+                frag_bf[u] = make_uint4(hw[0], hw[1], hw[2], hw[3]);
+                frag_bf[u + 64] = make_uint4(lw[0], lw[1], lw[2], lw[3]);
+            }
         norms_perm[p] = j >= 0 ? nrm : __builtin_inff();
-        if (j >= 0) atomicMax(nwmax_bits, __builtin_bit_cast(unsigned int, nrm));
+        if (j >= 0) {
+#pragma unroll
+            for (int d = 0; d < DP; d++) am[d] = fabsf(v[d]);
+            nmax = nrm;
+        }
     }
+#pragma unroll
+    for (int d = 0; d < DP; d++) {
+        float m = am[d];
+#pragma unroll
+        for (int o = WAVE / 2; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, WAVE));
+        if (lane == 0 && m > 0.f) atomicMax(&wmax_bits[d], __builtin_bit_cast(unsigned int, m));
+    }
+#pragma unroll
+    for (int o = WAVE / 2; o > 0; o >>= 1) nmax = fmaxf(nmax, __shfl_xor(nmax, o, WAVE));
+    if (lane == 0 && nmax > 0.f) atomicMax(nwmax_bits, __builtin_bit_cast(unsigned int, nmax));
     // natural slot -> FP32 fragments and natural norms
     if (p < nblk * 32) {
         const int j = p < mn ? p : -1;
+        float v[DP];
         float nrm = 0.f;
-        const int blk = p >> 5, row = p & 31;
-        for (int d = 0; d < dpad; d++) {
-            float v = (j >= 0 && d < dim) ? __fsub_rn(w[(int64_t)j * dim + d], centre[d]) : 0.f;
-            nrm = __fmaf_rn(v, v, nrm);
-            const int st = d >> 1, half = d & 1;
-            const int lane = row + 32 * half;
-            frag_f32[(((int64_t)blk * ks4 + (st >> 2)) * 64 + lane) * 4 + (st & 3)] = v;
+#pragma unroll
+        for (int d = 0; d < DP; d++) {
+            v[d] = (j >= 0 && d < dim) ? __fsub_rn(w[(int64_t)j * dim + d], centre[d]) : 0.f;
+            nrm = __fmaf_rn(v[d], v[d], nrm);
+        }
+        // MFMA 32x32x2 operand order: K step st = d/2 goes to lane row + 32*(d&1), element st&3
+#pragma unroll
+        for (int q = 0; q < DP / 8; q++) {
+            const int64_t u = (((int64_t)blk * (DP / 8) + q) * 64 + row);
+            frag_f32[u] = f32x4{v[8 * q], v[8 * q + 2], v[8 * q + 4], v[8 * q + 6]};
+            frag_f32[u + 32] = f32x4{v[8 * q + 1], v[8 * q + 3], v[8 * q + 5], v[8 * q + 7]};
         }
         norms_nat[p] = j >= 0 ? nrm : __builtin_inff();
     }
@@ -914,13 +967,21 @@ int somhip_prepare(somhip_model *M)
     if (M->prepared) return SOMHIP_OK;
     hipStream_t st = M->ctx->stream;
     TimerScope ts(M->ctx, TK_PREP);
-    hipLaunchKernelGGL(prep_centre_kernel, dim3(M->dpad), dim3(256), 0, st, M->w, M->mn, M->dim, M->dpad, M->centre);
-    SOM_HIP_TRY(hipMemsetAsync(M->wmax, 0, sizeof(float) * M->dpad, st));
-    SOM_HIP_TRY(hipMemsetAsync(M->scal, 0, sizeof(float) * 4, st));
+    if (M->dpad != 16 && M->dpad != 32) {
+        somhip_set_error("MFMA precisions support dim <= 32");
+        return SOMHIP_ERR_UNSUPPORTED;
+    }
+    hipLaunchKernelGGL(prep_centre_kernel, dim3(M->dpad), dim3(256), 0, st, M->w, M->mn, M->dim, M->dpad, M->centre,
+                       M->wmax, M->scal);
     const int slots = std::max(M->nblk, M->pblk) * 32;
-    hipLaunchKernelGGL(prep_rows_kernel, dim3((slots + 255) / 256), dim3(256), 0, st, M->w, M->mn, M->dim, M->dpad,
-                       M->nblk, M->pblk, M->centre, M->perm, M->frag_bf, M->frag_f32, M->wslot, M->norms_perm, M->norms_nat,
-                       (unsigned int *)M->wmax, (unsigned int *)M->scal);
+#define SOM_PREP(D)                                                                                           \
+    hipLaunchKernelGGL(prep_rows_kernel<D>, dim3((slots + 255) / 256), dim3(256), 0, st, M->w, M->mn, M->dim,  \
+                       M->nblk, M->pblk, M->centre, M->perm, (uint4 *)M->frag_bf, (f32x4 *)M->frag_f32,       \
+                       (f32x4 *)M->wslot, M->norms_perm, M->norms_nat, (unsigned int *)M->wmax,               \
+                       (unsigned int *)M->scal)
+    if (M->dpad == 16) SOM_PREP(16);
+    else SOM_PREP(32);
+#undef SOM_PREP
     SOM_HIP_TRY(hipGetLastError());
     M->prepared = true;
     return SOMHIP_OK;

@@ -252,8 +252,16 @@ __global__ void scatter_reduce_kernel(const unsigned long long *__restrict__ par
 {
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= width) return;
+    // integer sums are associative: 8 independent partial sums keep 8 loads in flight
+    unsigned long long s8[8] = {0ull, 0ull, 0ull, 0ull, 0ull, 0ull, 0ull, 0ull};
+    int b = 0;
+    for (; b + 8 <= nwg; b += 8)
+#pragma unroll
+        for (int u = 0; u < 8; u++) s8[u] += partial[(int64_t)(b + u) * width + e];
+    for (; b < nwg; b++) s8[0] += partial[(int64_t)b * width + e];
     unsigned long long s = 0ull;
-    for (int b = 0; b < nwg; b++) s += partial[(int64_t)b * width + e];
+#pragma unroll
+    for (int u = 0; u < 8; u++) s += s8[u];
     const int k = (int)(e % nv);
     acc[e] += ldexp((double)(long long)s, -shifts[k]);
 }
@@ -400,7 +408,7 @@ __global__ __launch_bounds__(256) void errors_partial_kernel(const int32_t *__re
                                                              int64_t n, const float *__restrict__ topo, double penalty,
                                                              int64_t per_wg, double *__restrict__ part)
 {
-    __shared__ double red[3][256];
+    __shared__ double red[4][3];
     const int64_t i0 = (int64_t)blockIdx.x * per_wg, i1 = min(n, i0 + per_wg);
     double q = 0.0, t = 0.0, s = 0.0;
     for (int64_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
@@ -412,14 +420,18 @@ __global__ __launch_bounds__(256) void errors_partial_kernel(const int32_t *__re
         if (sqd_ref32(&topo[2 * a], &topo[2 * b], 2) > penalty) t += r;
         s += r;
     }
-    red[0][threadIdx.x] = q; red[1][threadIdx.x] = t; red[2][threadIdx.x] = s;
-    __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-        if ((int)threadIdx.x < o)
-            for (int k = 0; k < 3; k++) red[k][threadIdx.x] += red[k][threadIdx.x + o];
-        __syncthreads();
+    // fixed-order reduction: butterfly within each wave, then the 4 waves in order
+    for (int o = 32; o > 0; o >>= 1) {
+        q += __shfl_xor(q, o, 64);
+        t += __shfl_xor(t, o, 64);
+        s += __shfl_xor(s, o, 64);
     }
-    if (threadIdx.x < 3) part[(int64_t)blockIdx.x * 3 + threadIdx.x] = red[threadIdx.x][0];
+    const int wv = threadIdx.x / 64;
+    if ((threadIdx.x & 63) == 0) { red[wv][0] = q; red[wv][1] = t; red[wv][2] = s; }
+    __syncthreads();
+    if (threadIdx.x < 3)
+        part[(int64_t)blockIdx.x * 3 + threadIdx.x] =
+            ((red[0][threadIdx.x] + red[1][threadIdx.x]) + red[2][threadIdx.x]) + red[3][threadIdx.x];
 }
 
 __global__ __launch_bounds__(256) void errors_final_kernel(const double *__restrict__ part, int nparts,
@@ -473,7 +485,7 @@ int somhip_launch_shifts_parts(hipStream_t st, const float *parts, int64_t rows,
                                const unsigned int *rho_bound, int64_t n, int *shifts)
 {
     // first stage: fold the rows in place into the first RB rows (RB x dpad maxima), then one block
-    constexpr int RB = 256;
+    constexpr int RB = 2048;
     if (rows > RB) {
         hipLaunchKernelGGL(rowmax_fold_kernel, dim3(RB), dim3(256), 0, st, (float *)parts, rows, dpad, RB);
         SOM_HIP_TRY(hipGetLastError());
