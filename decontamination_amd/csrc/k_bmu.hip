@@ -631,22 +631,46 @@ __global__ __launch_bounds__(NW_BF16 * WAVE) void bmu_bf16_tiles_kernel(const Bm
                                         (int64_t)blockIdx.x << FB_TILE_SHIFT, sm, nullptr);
 }
 
-// Exact top-2 of one sample by one wave: lane l scans neurons l, l+64, ...; the lanes' disjoint
-// top-2 lists are merged by a butterfly, lexicographic on (distance, index).
+// Exact top-2 of one sample by one workgroup of FB_THREADS threads: thread t scans the permuted
+// slots t, t + FB_THREADS, ... of the slot-ordered weight copy (4 rows in flight per thread), then
+// the top-2 lists are merged (butterfly in each wave, then the waves through LDS), lexicographic on
+// (distance, neuron index), so the scan order does not matter. Uniform over the workgroup.
+constexpr int FB_THREADS = 256;
+
+struct Top2Slot {
+    double d1, d2;
+    int i1, i2;
+};
+
 template <int DP, int MODE>
-__device__ __forceinline__ Top2 wave_exact_top2(const float *__restrict__ xrow, int dim, const float *__restrict__ w,
-                                                int mn, int lane)
+__device__ __forceinline__ Top2 group_exact_top2(const float *__restrict__ xrow, int dim, const BmuModelDev &Md,
+                                                 Top2Slot *red)
 {
+    constexpr int U = 4;
+    const int tid = threadIdx.x;
     float xr[DP];
 #pragma unroll
     for (int k = 0; k < DP; k++) xr[k] = k < dim ? xrow[k] : 0.f;
     Top2 tp;
     tp.init();
-    for (int j = lane; j < mn; j += WAVE) {
-        float wr[DP];
+    const int nslots = Md.nblk * 32;
+    const f32x4 *__restrict__ rows = (const f32x4 *)Md.wslot;
+    for (int p0 = tid; p0 < nslots; p0 += U * FB_THREADS) {
+        float wr[U][DP];
+        int jj[U];
 #pragma unroll
-        for (int k = 0; k < DP; k++) wr[k] = k < dim ? w[(int64_t)j * dim + k] : 0.f;
-        tp.push(sqd_mode<MODE>(wr, xr, DP), j);
+        for (int u = 0; u < U; u++) {
+            const int p = p0 + u * FB_THREADS;
+            jj[u] = p < nslots ? Md.perm[p] : -1;
+#pragma unroll
+            for (int q = 0; q < DP / 4; q++) {
+                const f32x4 t = p < nslots ? rows[(int64_t)p * (DP / 4) + q] : f32x4{0.f, 0.f, 0.f, 0.f};
+                wr[u][4 * q] = t[0]; wr[u][4 * q + 1] = t[1]; wr[u][4 * q + 2] = t[2]; wr[u][4 * q + 3] = t[3];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++)
+            if (jj[u] >= 0) tp.push(sqd_mode<MODE>(wr[u], xr, DP), jj[u]);  // zero-padded dims add zeros
     }
 #pragma unroll
     for (int o = WAVE / 2; o > 0; o >>= 1) {
@@ -655,25 +679,33 @@ __device__ __forceinline__ Top2 wave_exact_top2(const float *__restrict__ xrow, 
         if (pi1 >= 0) tp.push(pd1, pi1);
         if (pi2 >= 0) tp.push(pd2, pi2);
     }
-    return tp;
+    if ((tid & (WAVE - 1)) == 0) red[tid / WAVE] = Top2Slot{tp.d1, tp.d2, tp.i1, tp.i2};
+    __syncthreads();
+    if (tid == 0)
+        for (int v = 1; v < FB_THREADS / WAVE; v++) {
+            if (red[v].i1 >= 0) tp.push(red[v].d1, red[v].i1);
+            if (red[v].i2 >= 0) tp.push(red[v].d2, red[v].i2);
+        }
+    __syncthreads();  // red is reused by the next sample
+    return tp;        // complete in thread 0
 }
 
-// Fallback of the plain BF16 form: one wave per listed sample, grid-stride over the device-side count.
+// Fallback of the plain BF16 form: one workgroup per listed sample, grid-stride over the device-side
+// count.
 template <int DP, int MODE>
-__global__ __launch_bounds__(256) void bmu_exact_wave_kernel(const float *__restrict__ x, int dim,
-                                                             const int64_t *__restrict__ list,
-                                                             const unsigned int *__restrict__ list_count,
-                                                             const float *__restrict__ w, int mn, int default_index,
-                                                             int32_t *__restrict__ c1, int32_t *__restrict__ c2,
-                                                             double *__restrict__ d1)
+__global__ __launch_bounds__(FB_THREADS) void bmu_exact_group_kernel(BmuModelDev Md, const float *__restrict__ x,
+                                                                     const int64_t *__restrict__ list,
+                                                                     const unsigned int *__restrict__ list_count,
+                                                                     int default_index, int32_t *__restrict__ c1,
+                                                                     int32_t *__restrict__ c2,
+                                                                     double *__restrict__ d1)
 {
-    const int lane = threadIdx.x & (WAVE - 1);
-    const int64_t nw = (int64_t)gridDim.x * (blockDim.x / WAVE);
+    __shared__ Top2Slot red[FB_THREADS / WAVE];
     const int64_t total = (int64_t)(*list_count);
-    for (int64_t t = (int64_t)blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE; t < total; t += nw) {
+    for (int64_t t = blockIdx.x; t < total; t += gridDim.x) {
         const int64_t i = list[t];
-        const Top2 tp = wave_exact_top2<DP, MODE>(x + i * dim, dim, w, mn, lane);
-        if (lane == 0) {
+        const Top2 tp = group_exact_top2<DP, MODE>(x + i * Md.dim, Md.dim, Md, red);
+        if (threadIdx.x == 0) {
             c1[i] = tp.i1 >= 0 ? tp.i1 : default_index;
             if (c2) c2[i] = tp.i2;
             if (d1) d1[i] = tp.i1 >= 0 ? tp.d1 : __builtin_nan("");
@@ -681,24 +713,23 @@ __global__ __launch_bounds__(256) void bmu_exact_wave_kernel(const float *__rest
     }
 }
 
-// Fallback of the serving form: one wave per (tile, row) entry of the list (entries of one launch
+// Fallback of the serving form: one workgroup per (tile, row) entry of the list (entries of one launch
 // may belong to different models).
 template <int DP, int MODE>
-__global__ __launch_bounds__(256) void bmu_exact_tiles_kernel(const BmuModelDev *__restrict__ models,
-                                                             const BmuTile *__restrict__ tiles,
-                                                             const int64_t *__restrict__ list,
-                                                             const unsigned int *__restrict__ list_count)
+__global__ __launch_bounds__(FB_THREADS) void bmu_exact_tiles_kernel(const BmuModelDev *__restrict__ models,
+                                                                     const BmuTile *__restrict__ tiles,
+                                                                     const int64_t *__restrict__ list,
+                                                                     const unsigned int *__restrict__ list_count)
 {
-    const int lane = threadIdx.x & (WAVE - 1);
-    const int64_t nw = (int64_t)gridDim.x * (blockDim.x / WAVE);
+    __shared__ Top2Slot red[FB_THREADS / WAVE];
     const int64_t total = (int64_t)(*list_count);
-    for (int64_t t = (int64_t)blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE; t < total; t += nw) {
+    for (int64_t t = blockIdx.x; t < total; t += gridDim.x) {
         const int64_t e = list[t];
         const BmuTile T = tiles[e >> FB_TILE_SHIFT];
         const int64_t i = e & ((1ll << FB_TILE_SHIFT) - 1);
         const BmuModelDev Md = models[T.model];
-        const Top2 tp = wave_exact_top2<DP, MODE>(T.x + i * Md.dim, Md.dim, Md.w, Md.mn, lane);
-        if (lane == 0) {
+        const Top2 tp = group_exact_top2<DP, MODE>(T.x + i * Md.dim, Md.dim, Md, red);
+        if (threadIdx.x == 0) {
             T.c1[i] = tp.i1 >= 0 ? tp.i1 : T.default_index;
             if (T.c2) T.c2[i] = tp.i2;
             if (T.d1) T.d1[i] = tp.i1 >= 0 ? tp.d1 : __builtin_nan("");
@@ -854,16 +885,16 @@ void launch_exact(int mode, bool need_c2, hipStream_t st, const float *x, int64_
 }
 
 template <int DP>
-void launch_exact_wave(int mode, hipStream_t st, const float *x, int dim, const int64_t *list,
-                       const unsigned int *list_count, int64_t blocks, const float *w, int mn, int default_index,
-                       int32_t *c1, int32_t *c2, double *d1)
+void launch_exact_group(int mode, hipStream_t st, const BmuModelDev &Md, const float *x, const int64_t *list,
+                        const unsigned int *list_count, int64_t blocks, int default_index, int32_t *c1, int32_t *c2,
+                        double *d1)
 {
     if (mode == SOMHIP_RERANK_FP64)
-        hipLaunchKernelGGL((bmu_exact_wave_kernel<DP, SOMHIP_RERANK_FP64>), dim3((unsigned)blocks), dim3(256), 0, st,
-                           x, dim, list, list_count, w, mn, default_index, c1, c2, d1);
+        hipLaunchKernelGGL((bmu_exact_group_kernel<DP, SOMHIP_RERANK_FP64>), dim3((unsigned)blocks), dim3(FB_THREADS),
+                           0, st, Md, x, list, list_count, default_index, c1, c2, d1);
     else
-        hipLaunchKernelGGL((bmu_exact_wave_kernel<DP, SOMHIP_RERANK_REF32>), dim3((unsigned)blocks), dim3(256), 0, st,
-                           x, dim, list, list_count, w, mn, default_index, c1, c2, d1);
+        hipLaunchKernelGGL((bmu_exact_group_kernel<DP, SOMHIP_RERANK_REF32>), dim3((unsigned)blocks), dim3(FB_THREADS),
+                           0, st, Md, x, list, list_count, default_index, c1, c2, d1);
 }
 
 constexpr int G_FP32 = 2;
@@ -1065,11 +1096,12 @@ int somhip_launch_bmu(somhip_model *M, const float *x, int64_t n, int precision,
         SOM_HIP_TRY(hipGetLastError());
     }
     TimerScope tf(M->ctx, TK_BMU_FALLBACK);
-    // Fallback over the flagged samples: one wave per sample, a fixed grid that reads the device-side
-    // count (an empty list costs one launch).
-    const int64_t fb_blocks = std::max<int64_t>(1, std::min<int64_t>((n + 3) / 4, (int64_t)M->ctx->num_cus * 8));
-    if (dp == 16) launch_exact_wave<16>(arithmetic, st, x, M->dim, fb_list, fb_count, fb_blocks, M->w, M->mn, default_index, c1, c2, d1);
-    else launch_exact_wave<32>(arithmetic, st, x, M->dim, fb_list, fb_count, fb_blocks, M->w, M->mn, default_index, c1, c2, d1);
+    // Fallback over the flagged samples: one workgroup per sample, a fixed grid that reads the
+    // device-side count (an empty list costs one launch).
+    const int64_t fb_blocks = std::max<int64_t>(1, std::min<int64_t>(n, (int64_t)M->ctx->num_cus * 8));
+    const BmuModelDev Md = somhip_model_dev(M);
+    if (dp == 16) launch_exact_group<16>(arithmetic, st, Md, x, fb_list, fb_count, fb_blocks, default_index, c1, c2, d1);
+    else launch_exact_group<32>(arithmetic, st, Md, x, fb_list, fb_count, fb_blocks, default_index, c1, c2, d1);
     SOM_HIP_TRY(hipGetLastError());
     return SOMHIP_OK;
 }
