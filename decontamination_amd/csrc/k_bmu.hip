@@ -437,14 +437,20 @@ __device__ __forceinline__ void bmu_bf16_body(const BmuModelDev &Md, const float
     };
     // Streams chunks [c0, c1) through the LDS buffers; per_block(blk, acc) sees every block's D values,
     // after_chunk() runs once per chunk, after the chunk's closing barrier.
-    auto stream = [&](int c0, int c1, auto &&per_block, auto &&after_chunk) {
-        __syncthreads();  // the buffers may still be read by the previous phase
-        load_chunk(c0);
-        store_chunk(0);
-        __syncthreads();
+    // The buffer in use alternates across phases too; a phase's last chunk prefetches the first chunk
+    // of the next phase (next_c0 >= 0), so only the first phase waits for its first load.
+    int buf = 0;
+    bool preloaded = false;
+    auto stream = [&](int c0, int c1, int next_c0, auto &&per_block, auto &&after_chunk) {
+        if (!preloaded) {
+            __syncthreads();  // the buffers may still be read by the previous phase
+            load_chunk(c0);
+            store_chunk(buf);
+            __syncthreads();
+        }
         for (int c = c0; c < c1; c++) {
-            const int buf = (c - c0) & 1;
-            if (c + 1 < c1) load_chunk(c + 1);
+            const int nc = c + 1 < c1 ? c + 1 : next_c0;
+            if (nc >= 0) load_chunk(nc);
             const int b0 = c * CB;
             const int b1 = min(nblk, b0 + CB);
             for (int blk = b0; blk < b1; blk++) {
@@ -456,10 +462,12 @@ __device__ __forceinline__ void bmu_bf16_body(const BmuModelDev &Md, const float
                 for (int g = 0; g < G; g++) acc[g] = block_acc(g, ah, al, cin);
                 per_block(blk, acc);
             }
-            if (c + 1 < c1) store_chunk(buf ^ 1);
+            if (nc >= 0) store_chunk(buf ^ 1);
             __syncthreads();
+            buf ^= 1;
             after_chunk();
         }
+        preloaded = next_c0 >= 0;
     };
     auto block_min = [](const f32x16 &v) {
         float t = __builtin_inff();
@@ -518,7 +526,7 @@ __device__ __forceinline__ void bmu_bf16_body(const BmuModelDev &Md, const float
     const int nchunks_a = (Md.nblk_a + CB - 1) / CB;
 
     // ---------------- phase A: coarse section, top-2 only ----------------
-    stream(0, nchunks_a, [&](int, f32x16 (&acc)[G]) {
+    stream(0, nchunks_a, nchunks_a < nchunks ? nchunks_a : 0, [&](int, f32x16 (&acc)[G]) {
 #pragma unroll
         for (int g = 0; g < G; g++) update(g, block_min(acc[g]));
     }, nothing);
@@ -529,7 +537,7 @@ __device__ __forceinline__ void bmu_bf16_body(const BmuModelDev &Md, const float
     // threshold in force at any time is >= the final one, so the recorded set contains every
     // neuron with D <= final T.
     if (nchunks_a < nchunks) {
-        stream(nchunks_a, nchunks, [&](int blk, f32x16 (&acc)[G]) {
+        stream(nchunks_a, nchunks, 0, [&](int blk, f32x16 (&acc)[G]) {
 #pragma unroll
             for (int g = 0; g < G; g++) {
                 const float t = block_min(acc[g]);
@@ -540,7 +548,7 @@ __device__ __forceinline__ void bmu_bf16_body(const BmuModelDev &Md, const float
     }
 
     // ---------------- phase C: the coarse section again, recording under the final threshold ----------------
-    stream(0, nchunks_a, [&](int blk, f32x16 (&acc)[G]) {
+    stream(0, nchunks_a, -1, [&](int blk, f32x16 (&acc)[G]) {
 #pragma unroll
         for (int g = 0; g < G; g++) record(blk, g, acc[g], block_min(acc[g]));
     }, nothing);
