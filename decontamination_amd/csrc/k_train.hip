@@ -610,7 +610,8 @@ int somhip_launch_iter(hipStream_t st, const int32_t *bmu, const float *x, int d
     return SOMHIP_OK;
 }
 
-int somhip_errors_nparts(int64_t n) { return (int)std::max<int64_t>(1, std::min<int64_t>(4096, (n + 4095) / 4096)); }
+// about 1024 samples (4 per thread) per workgroup: enough workgroups to keep the loads in flight
+int somhip_errors_nparts(int64_t n) { return (int)std::max<int64_t>(1, std::min<int64_t>(65536, (n + 1023) / 1024)); }
 
 int somhip_launch_errors(hipStream_t st, const int32_t *c1, const int32_t *c2, const double *d1, const double *rho,
                          int64_t n, const float *topo, double penalty, double *part, int nparts, double *errors)

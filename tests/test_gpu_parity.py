@@ -166,6 +166,25 @@ def test_activation_vs_oracle(case, precision):
     assert np.allclose(got, exp.astype(np.float32), rtol=1e-6, atol=0)
 
 
+@pytest.mark.parametrize("shape", [(40, 40, 16), (30, 35, 32)])
+def test_bf16_window_overflow_falls_back_exactly(shape):
+    """A map whose neurons are mostly exact duplicates puts hundreds of neurons inside every sample's
+    window: the per-lane candidate lists overflow and every such sample goes through the exact
+    fallback, which must return the lowest-index tie like the oracle."""
+    m, n, dim = shape
+    rng = np.random.default_rng(17)
+    W = np.repeat(rng.standard_normal((4, dim)).astype(np.float32), m * n // 4 + 1, axis=0)[: m * n]
+    W[::7] += rng.standard_normal((len(W[::7]), dim)).astype(np.float32)
+    X = rng.standard_normal((3000, dim)).astype(np.float32)
+    for precision in ("bf16_rerank",):
+        som = dam.SOM_Batch(m, n, dim, precision=precision)
+        som._weights[...] = W
+        assert np.array_equal(som.get_winners(X), R.find_bmus(W, X, REF32))
+        q, t = som.compute_errors(X)
+        eq, et = R.compute_errors(W, R.topography(m, n), X, None, "hexagonal", REF32)
+        assert np.isclose(q, eq, rtol=1e-12, atol=0) and np.isclose(t, et, rtol=1e-12, atol=0)
+
+
 @pytest.mark.parametrize("precision", EXACT_PRECS)
 def test_fp64_arithmetic_matches_fp64_oracle(precision):
     W, X = make_case(12, 12, 16, 5000, 21)
