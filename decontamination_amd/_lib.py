@@ -62,6 +62,9 @@ SIGNATURES = {
     "somhip_memcpy_d2h": [P, P, P, SZ],
     "somhip_memcpy_d2d": [P, P, P, SZ],
     "somhip_memset": [P, P, I, SZ],
+    "somhip_uploader_create": [P, I, PP],
+    "somhip_uploader_destroy": [P],
+    "somhip_uploader_put": [P, P, SZ, P, SZ, PP, PP],
     "somhip_model_create": [P, I, I, I, I, P, PP],
     "somhip_model_destroy": [P],
     "somhip_model_set_weights": [P, P, I],

@@ -135,6 +135,44 @@ class DeviceArray:
             pass
 
 
+class Uploader:
+    """Ring of device slots for the chunk protocol (somhip_uploader_*): ``put`` copies a chunk on a
+    copy stream of its own while the kernels of the previous chunk run, and orders the context's
+    stream after the copy. The host arrays of the last ``slots`` chunks are kept referenced, so a
+    pinned buffer is not released while its copy may still be in flight."""
+
+    def __init__(self, ctx, slots=2):
+        h = ctypes.c_void_p()
+        call("somhip_uploader_create", ctx.handle, int(slots), ctypes.byref(h))
+        self.ctx = ctx
+        self.handle = h
+        self.slots = int(slots)
+        self._held = [None] * self.slots
+        self._next = 0
+
+    def put(self, x, rho=None):
+        """Device pointers (x_ptr, rho_ptr or None) of the uploaded chunk."""
+        xd = ctypes.c_void_p()
+        rd = ctypes.c_void_p()
+        call("somhip_uploader_put", self.handle, vp(x), x.nbytes, None if rho is None else vp(rho),
+             0 if rho is None else rho.nbytes, ctypes.byref(xd), ctypes.byref(rd))
+        self._held[self._next] = (x, rho)
+        self._next = (self._next + 1) % self.slots
+        return xd.value, (None if rho is None else rd.value)
+
+    def close(self):
+        if self.handle:
+            _lib.load().somhip_uploader_destroy(self.handle)
+            self.handle = None
+            self._held = [None] * self.slots
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class Model:
     """Weights (mn, dim) FP32 + topography (mn, 2) FP32 resident on the device."""
 

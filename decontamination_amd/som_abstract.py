@@ -20,7 +20,7 @@ import typing
 import numpy as np
 
 from . import _lib
-from .runtime import DeviceArray, Model, get_context
+from .runtime import DeviceArray, Model, Uploader, get_context
 
 
 def dataset_to_generator_builder(dataset):
@@ -43,30 +43,6 @@ def iterate_chunks(dataset, dataset_weights):
             if v.shape[0] != w.shape[0]:
                 raise ValueError('`dataset` and `dataset_weights` chunks must be aligned')
             yield v, w
-
-
-class _Staging:
-    """Grow-only device buffers for uploading chunks."""
-
-    def __init__(self, ctx):
-        self.ctx = ctx
-        self.bufs = {}
-
-    def get(self, key, nbytes):
-        b = self.bufs.get(key)
-        if b is None or b.nbytes < nbytes:
-            if b is not None:
-                b.free()
-            b = DeviceArray(self.ctx, (max(int(nbytes), 1),), np.uint8)
-            self.bufs[key] = b
-        return b
-
-    def upload(self, key, arr):
-        arr = np.ascontiguousarray(arr)
-        b = self.get(key, arr.nbytes)
-        if arr.nbytes:
-            _lib.call("somhip_memcpy_h2d", self.ctx.handle, _lib.vp(b.ptr), _lib.vp(arr), arr.nbytes)
-        return b.ptr
 
 
 # noinspection PyPep8Naming
@@ -322,19 +298,20 @@ class SOM_Abstract(object):
 
     def _stage(self):
         if self._staging is None:
-            self._staging = _Staging(self._ctx())
+            self._staging = Uploader(self._ctx(), slots=2)
         return self._staging
 
-    def _upload_chunk(self, vectors, weights, rho_key='rho'):
-        st = self._stage()
+    def _upload_chunk(self, vectors, weights):
+        """Device pointers of one chunk; its copy overlaps the kernels of the previous chunk."""
         x = np.ascontiguousarray(vectors, dtype=self._dtype)
         if x.ndim != 2 or x.shape[1] != self._dim:
             raise ValueError(f'dataset chunks must have shape (N, {self._dim})')
-        xp = st.upload('x', x)
-        rp = None
+        r = None
         if weights is not None:
             r = np.ascontiguousarray(weights, dtype=np.float64).reshape(-1)
-            rp = st.upload(rho_key, r)
+            if r.shape[0] != x.shape[0]:
+                raise ValueError('`dataset` and `dataset_weights` chunks must be aligned')
+        xp, rp = self._stage().put(x, r)
         return xp, rp, x.shape[0]
 
     # -- errors (som_abstract.py:528-601) -------------------------------------------------------

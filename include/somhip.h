@@ -94,6 +94,21 @@ int somhip_memcpy_d2h(somhip_ctx *ctx, void *dst, const void *src, size_t bytes)
 int somhip_memcpy_d2d(somhip_ctx *ctx, void *dst, const void *src, size_t bytes);
 int somhip_memset(somhip_ctx *ctx, void *dptr, int value, size_t bytes);
 
+/* Chunk uploader for the generator protocol (decontamination/algo/__init__.py:21-27; chunk loops
+ * som_batch.py:295-342, som_abstract.py:562-593, 632-664), replacing the synchronous per-launch
+ * copies of decontamination/jit/__init__.py:372-373. A ring of `slots` (2..8) device slots is filled
+ * on the uploader's own copy stream: put() copies one chunk (x and optional rho) into the next
+ * slot, after the kernels that read that slot last time round have finished, and makes the
+ * context's stream wait for the copy. The copy of chunk k+1 thus overlaps the kernels of chunk k.
+ * The returned device pointers stay valid until `slots` further puts. Pageable host memory may be
+ * reused once put() returns; memory from somhip_host_alloc must stay untouched until the
+ * context's stream has passed the kernels that read the chunk. */
+typedef struct somhip_uploader somhip_uploader;
+int somhip_uploader_create(somhip_ctx *ctx, int slots, somhip_uploader **out);
+int somhip_uploader_destroy(somhip_uploader *up);
+int somhip_uploader_put(somhip_uploader *up, const void *x_host, size_t x_bytes, const void *rho_host,
+                        size_t rho_bytes, void **x_dev, void **rho_dev);
+
 /* Model: weights (mn, dim) FP32 and topography (mn, 2) FP32 resident on the device.
  * Replaces SOM_Abstract._weights/_topography being copied to the device on every launch
  * (decontamination/jit/__init__.py:372-373). topography_host must be the reference's FP32
@@ -167,7 +182,7 @@ int somhip_batch_iter_update(somhip_accum *acc, float *history_out);
  *
  * A server keeps models resident, admits requests while their working memory fits in
  * `memory_budget` bytes of device memory (one slab allocated at creation), uploads their rows on a
- * copy stream, and batches the 256-row tiles of every uploaded request into shared launches
+ * copy stream, and batches the 512-row tiles of every uploaded request into shared launches
  * (hipGraphs captured per dimension class and tile-count bucket). Results are bit-identical to the
  * single-request entry points, alone or in any mix. A scheduler thread owns the server's streams.
  * Host buffers passed to submit must stay valid until the request completes; pinned buffers from

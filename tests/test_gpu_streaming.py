@@ -1,0 +1,140 @@
+"""GPU: chunked datasets through the uploader ring (somhip_uploader_*), whose copies overlap the
+kernels of the previous chunk.
+
+Bars: BMU indices bit-exact against the oracle per chunk, activation counts exact, and chunked
+training bit-identical to the same chunks uploaded synchronously one at a time (the overlap must not
+change a single bit), across ragged chunk sizes (empty, tiny, growing, shrinking) and a generator
+that reuses one host buffer for every chunk.
+"""
+import numpy as np
+import pytest
+
+import decontamination_amd as dam
+from decontamination_amd import _lib
+from decontamination_amd.runtime import Accumulator, DeviceArray, Model, Uploader, get_context
+from oracle import FP64, REF32, build as build_oracle
+from oracle import som_ref as R
+
+pytestmark = pytest.mark.gpu
+
+SIZES = [1000, 50_000, 3, 0, 120_000, 70_000, 160_000, 17, 9_000]
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _setup():
+    build_oracle()
+    assert dam.gpu_available(), "no GPU visible: the GPU tests must run on an MI355X"
+
+
+def _chunks(seed, dim):
+    rng = np.random.default_rng(seed)
+    xs = [rng.standard_normal((s, dim)).astype(np.float32) for s in SIZES]
+    rs = [rng.random(s) for s in SIZES]
+    return xs, rs
+
+
+@pytest.mark.parametrize("precision", ["exact", "bf16_rerank"])
+@pytest.mark.parametrize("slots", [2, 3])
+def test_uploader_ring_winners_and_counts(precision, slots):
+    m, n, dim = 16, 16, 16
+    print(f"uploader ring {precision} slots={slots}", flush=True)
+    rng = np.random.default_rng(7)
+    W = rng.standard_normal((m * n, dim)).astype(np.float32)
+    xs, _ = _chunks(8, dim)
+    ctx = get_context(0)
+    model = Model(ctx, m, n, dim, "hexagonal", R.topography(m, n, "hexagonal"))
+    model.set_weights(W)
+    prec = _lib.PRECISIONS[precision]
+    up = Uploader(ctx, slots=slots)
+    outs = [DeviceArray.empty(ctx, (max(len(x), 1),), np.int32) for x in xs]
+    act = DeviceArray.zeros(ctx, (m * n,), np.float64)
+    for x, o in zip(xs, outs):
+        xp, _ = up.put(x)
+        model.find_bmus(xp, len(x), o.ptr, prec, _lib.RERANK_REF32)
+        model.count_bmus(xp, None, len(x), act.ptr, prec, _lib.RERANK_REF32)
+    ctx.synchronize()
+    total = np.zeros(m * n)
+    for x, o in zip(xs, outs):
+        want = R.find_bmus(W, x, REF32)
+        got = o.copy_to_host()[:len(x)]
+        assert np.array_equal(got, want)
+        total += np.bincount(want, minlength=m * n)
+    assert np.array_equal(act.copy_to_host(), total)
+    up.close()
+
+
+def test_reused_host_buffer():
+    """A generator that overwrites one host array for every chunk: each chunk must be the one seen."""
+    m, n, dim = 12, 12, 8
+    rng = np.random.default_rng(21)
+    W = rng.standard_normal((m * n, dim)).astype(np.float32)
+    xs, _ = _chunks(22, dim)
+    buf = np.empty((max(SIZES), dim), np.float32)
+
+    def builder():
+        def gen():
+            for x in xs:
+                buf[:len(x)] = x
+                yield buf[:len(x)]
+        return gen
+
+    som = dam.SOM_Batch(m, n, dim, topology="square")
+    som._weights[...] = W
+    act = som.get_activation_map(lambda: builder())
+    want = R.activation_map(W, np.concatenate(xs), None, REF32, m, n)
+    assert np.array_equal(act, np.asarray(want, np.float32).reshape(m, n))
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16_rerank"])
+def test_streamed_training_equals_synchronous(precision):
+    m, n, dim, epochs = 20, 20, 16, 3
+    print(f"streamed training {precision}", flush=True)
+    rng = np.random.default_rng(31)
+    W0 = rng.standard_normal((m * n, dim)).astype(np.float32)
+    xs, rs = _chunks(32, dim)
+    gen = lambda arrs: (lambda: lambda: iter(arrs))
+
+    som = dam.SOM_Batch(m, n, dim, precision=precision)
+    som._weights[...] = W0
+    som.train(gen(xs), gen(rs), n_epochs=epochs, use_best_epoch=False)
+
+    # the same epochs with one synchronous upload per chunk and no overlap
+    ctx = get_context(0)
+    model = Model(ctx, m, n, dim, "hexagonal", R.topography(m, n, "hexagonal"))
+    model.set_weights(W0)
+    acc = Accumulator(model)
+    prec = _lib.PRECISIONS[precision]
+    hist = DeviceArray.empty(ctx, (m * n, dim), np.float32)
+    errors = DeviceArray.zeros(ctx, (epochs, 3), np.float64)
+    for e in range(epochs):
+        acc.reset()
+        for x, r in zip(xs, rs):
+            dx = DeviceArray.from_host(ctx, x) if len(x) else None
+            dr = DeviceArray.from_host(ctx, r) if len(r) else None
+            if len(x):
+                acc.epoch_accumulate(dx.ptr, dr.ptr, len(x), prec, _lib.RERANK_REF32,
+                                     errors.view_ptr((e - 1) * 3) if e else None)
+            ctx.synchronize()
+        acc.epoch_update(e, epochs, som.sigma, hist.ptr)
+    want = hist.copy_to_host()
+    assert np.array_equal(som.weights, want)
+
+
+def test_streamed_training_matches_oracle():
+    """Ragged weighted chunks against the oracle's chunked training in FP64 (FP64 distances and
+    accumulators, so the comparison is not limited by the reference's FP32 accumulators over 23k rows):
+    equal to about one FP32 ulp."""
+    m, n, dim, epochs = 8, 8, 8, 3
+    rng = np.random.default_rng(41)
+    sizes = [700, 0, 5, 9000, 2300, 11_000, 1]
+    xs = [rng.standard_normal((s, dim)).astype(np.float32) for s in sizes]
+    rs = [rng.random(s) for s in sizes]
+    W0 = rng.standard_normal((m * n, dim)).astype(np.float32)
+    gen = lambda arrs: (lambda: lambda: iter(arrs))
+    som = dam.SOM_Batch(m, n, dim, arithmetic="fp64")
+    som._weights[...] = W0
+    som.train(gen(xs), gen(rs), n_epochs=epochs, use_best_epoch=False)
+    Wr, _, qe, te = R.train(W0, m, n, gen(xs), gen(rs), n_epochs=epochs, use_best_epoch=False, mode=FP64)
+    assert np.allclose(som.weights, Wr, rtol=1e-6, atol=1e-7)
+    assert np.allclose(som.quantization_errors, qe, rtol=1e-6)
+    assert np.allclose(som.topographic_errors, te, rtol=1e-9, atol=0)
