@@ -73,6 +73,8 @@ SIGNATURES = {
     "somhip_find_bmus": [P, P, I64, I, I, P],
     "somhip_count_bmus": [P, P, P, I64, I, I, P],
     "somhip_compute_errors": [P, P, P, I64, I, I, P],
+    "somhip_gndm": [P, P, P, I64, I, P, P, P, P],
+    "somhip_covariance_chunk": [P, P, I, P, I64, I, P],
     "somhip_accum_create": [P, PP],
     "somhip_accum_destroy": [P],
     "somhip_accum_reset": [P],

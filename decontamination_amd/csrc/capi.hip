@@ -7,34 +7,11 @@
 #include <string>
 #include <vector>
 
-// launchers from k_train.hip
-int somhip_launch_absmax(hipStream_t st, const float *x, int dim, int dpad, const double *rho, int rho_f32, int64_t n,
-                         unsigned int *bounds, int num_cus);
-int somhip_launch_shifts(hipStream_t st, const unsigned int *bounds, int dim, int has_x, int has_rho, int64_t n,
-                         int *shifts);
-int somhip_launch_shifts_parts(hipStream_t st, const float *parts, int64_t rows, int dpad, int dim,
-                               const unsigned int *rho_bound, int64_t n, int *shifts);
-int somhip_scatter_nwg(int64_t n, int mn, int nv, int num_cus, size_t table_budget);
-int somhip_launch_scatter(hipStream_t st, const int32_t *bmu, const float *x, int dim, const double *rho, int rho_f32,
-                          int64_t n, const int *shifts, int mn, int nv, int nwg, unsigned long long *partial,
-                          double *acc, size_t lds_limit);
-int somhip_update_nsplit(int mn);
-int somhip_launch_update(hipStream_t st, const double *S, const float *topo, int mn, int dim, double sigma,
-                         double *part, int nsplit, float *w, float *hist);
-int somhip_launch_divide(hipStream_t st, const double *acc, int mn, int dim, float *w, float *hist);
-int somhip_iter_nsplit(int64_t n, int mn);
-int somhip_launch_iter(hipStream_t st, const int32_t *bmu, const float *x, int dim, int dpad, const double *rho,
-                       int64_t n, int64_t cur, int64_t nvec, float sigma0, const float *topo, int mn, double *part,
-                       int nsplit, double *acc);
-int somhip_errors_nparts(int64_t n);
-int somhip_launch_errors(hipStream_t st, const int32_t *c1, const int32_t *c2, const double *d1, const double *rho,
-                         int64_t n, const float *topo, double penalty, double *part, int nparts, double *errors);
+#include "launch.hpp"
 
 namespace {
 
 thread_local std::string g_last_error;
-
-constexpr size_t SCATTER_TABLE_BUDGET = (size_t)192 << 20;  // bytes of per-workgroup partial tables
 
 inline int check_model(const somhip_model *M)
 {

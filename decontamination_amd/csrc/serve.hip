@@ -27,15 +27,7 @@
 #include <unordered_map>
 #include <vector>
 
-// launchers from k_train.hip
-int somhip_launch_absmax(hipStream_t st, const float *x, int dim, int dpad, const double *rho, int rho_f32, int64_t n,
-                         unsigned int *bounds, int num_cus);
-int somhip_launch_shifts(hipStream_t st, const unsigned int *bounds, int dim, int has_x, int has_rho, int64_t n,
-                         int *shifts);
-int somhip_scatter_nwg(int64_t n, int mn, int nv, int num_cus, size_t table_budget);
-int somhip_launch_scatter(hipStream_t st, const int32_t *bmu, const float *x, int dim, const double *rho, int rho_f32,
-                          int64_t n, const int *shifts, int mn, int nv, int nwg, unsigned long long *partial,
-                          double *acc, size_t lds_limit);
+#include "launch.hpp"
 
 namespace {
 

@@ -142,6 +142,27 @@ int somhip_count_bmus(somhip_model *model, const float *x, const double *rho, in
 int somhip_compute_errors(somhip_model *model, const float *x, const double *rho, int64_t n, int precision,
                           int arithmetic, double *errors);
 
+/* Galaxy number density from the footprint winners: the two activation maps of
+ * Decontamination_SOM.compute_gndm (catalog: rho = coverage-corrected density, footprint: ones;
+ * rows with rho > 0 count, the _count_bmus_kernel rule) cast to FP32, gnd = catalog / footprint in
+ * FP32 where footprint > 0 and NaN elsewhere, and gndm[i] = gnd[winners[i]] (NaN for an index outside
+ * [0, mn)). Replaces the two get_activation_map passes and _compute_gndm
+ * (decontamination/decontamination/decontamination_som.py:600-667): the winners of the footprint
+ * are the BMUs both maps would search again. Device pointers: winners int32 (n) and gndm FP32 (n),
+ * both 16-byte aligned; rho FP64 (n); catalog_map, footprint_map, gnd FP32 (mn). */
+int somhip_gndm(somhip_ctx *ctx, const int32_t *winners, const double *rho, int64_t n, int mn, float *catalog_map,
+                float *footprint_map, float *gnd, float *gndm);
+
+/* Weighted covariance sums of one chunk for Covariance.compute (decontamination/algo/covariance.py,
+ * _update_welford_cov_sums and _finalize_welford_cov), the pass that initialises SOM_PCA. A row
+ * counts when its weight is finite and > 0 and all its values are finite (rho == NULL means ones).
+ * x: device (n, dim), FP32 or (x_is_f64) FP64; 1 <= dim <= 64. out_host (FP64, 1 + dim + dim*dim):
+ * total weight W, mean (dim), co-moment M2 = sum w (x - mean)(x - mean)^T (dim x dim). Two passes over
+ * the chunk with fixed-order reductions (deterministic); chunks merge with Chan's update, and the
+ * covariance is M2 / (W - ddof). Synchronises. */
+int somhip_covariance_chunk(somhip_ctx *ctx, const void *x, int x_is_f64, const double *rho, int64_t n, int dim,
+                            double *out_host);
+
 /* Training accumulator: per-neuron sums S_c = sum rho_i x_i and c_c = sum rho_i over the samples whose
  * BMU is c (epoch mode), or the neighbourhood-weighted numerator/denominator (vector-count mode).
  * Replaces the numerator/denominator DeviceArrays of SOM_Batch.train

@@ -57,6 +57,7 @@ def lib():
         "orc_divide": [I, P, P, I, I, P],
         "orc_train_epoch_accumulate_omp": [P, P, P, P, P, P, I64, I64, I64, F, I, I],
         "orc_compute_errors_omp": [P, P, I, I, P, P, I64, D, P],
+        "orc_welford_update": [P, P, P, P, P, P, I64, I],
     }
     for name, argtypes in sigs.items():
         fn = getattr(L, name)

@@ -20,6 +20,7 @@
  *   train epoch step    decontamination/algo/som_batch.py:94-150,509-566
  *   train iter step     decontamination/algo/som_batch.py:154-210,509-566
  *   sigma decay         decontamination/algo/__init__.py:64-74, clamp som_batch.py:107-110,133-136
+ *   welford update      decontamination/algo/covariance.py:30-84 (_update_welford_cov_sums)
  *
  * Pinned by: the reference's golden vectors (test/test_som_batch.py:30-89,
  * test/test_som_abstract.py:41-64), brute force on tiny inputs and closed forms (tests/test_oracle.py).
@@ -263,4 +264,33 @@ void orc_compute_errors_omp(const float *w, const float *topo, int mn, int dim, 
     errors[0] += e0;
     errors[1] += e1;
     errors[2] += e2;
+}
+
+/* Covariance.compute's sample loop (decontamination/algo/covariance.py:30-84, _update_welford_cov_sums):
+ * sequential weighted Welford update in FP64 over one chunk, skipping rows whose weight is not
+ * finite or <= 0 and rows with a non-finite value. x is FP64 (the reference casts each chunk with
+ * astype(np.float64)); w may be NULL (ones). Updates *sum_w, mean (dim) and the upper triangle of
+ * m2 (dim x dim, row-major); delta is scratch (dim). */
+void orc_welford_update(double *sum_w, double *mean, double *m2, double *delta, const double *x, const double *w,
+                        int64_t n, int dim)
+{
+    double sw = *sum_w;
+    for (int64_t i = 0; i < n; i++) {
+        const double wi = w ? w[i] : 1.0;
+        if (!isfinite(wi) || wi <= 0.0) continue;
+        const double *xi = x + i * dim;
+        int ok = 1;
+        for (int d = 0; d < dim; d++) ok &= isfinite(xi[d]) ? 1 : 0;
+        if (!ok) continue;
+        const double sw_new = sw + wi;
+        for (int d = 0; d < dim; d++) delta[d] = xi[d] - mean[d];
+        const double a = wi / sw_new;
+        for (int d = 0; d < dim; d++) mean[d] += a * delta[d];
+        for (int j = 0; j < dim; j++) {
+            const double dj = delta[j];
+            for (int k = j; k < dim; k++) m2[j * dim + k] += wi * dj * (xi[k] - mean[k]);
+        }
+        sw = sw_new;
+    }
+    *sum_w = sw;
 }

@@ -222,3 +222,73 @@ def factorised_epoch_fp64(W, topo, X, rho, sigma, bmu=None, out_dtype=np.float32
     den = H @ cnt
     Wn = np.where(den[:, None] > FLOAT_EPS, num / np.where(den[:, None] > FLOAT_EPS, den[:, None], 1.0), 0.0)
     return Wn.astype(out_dtype), c1, c2, d1
+
+
+# ---------------------------------------------------------------------------------------------------
+# Covariance / SOM_PCA / galaxy-number-density restatements
+# ---------------------------------------------------------------------------------------------------
+def covariance(dim, dataset, dataset_weights=None, ddof=1):
+    """Covariance.compute (covariance.py:124-212): sequential weighted Welford over the chunks
+    (orc_welford_update, covariance.py:30-84), then _finalize_welford_cov (covariance.py:90-119)."""
+    if ddof < 0:
+        raise ValueError('ddof must be greater than or equal to zero.')
+    sum_w = np.zeros(1, np.float64)
+    mean = np.zeros(dim, np.float64)
+    m2 = np.zeros((dim, dim), np.float64)
+    delta = np.empty(dim, np.float64)
+    dch = _chunks(dataset)
+    wch = _chunks(dataset_weights)
+    pairs = ((c, None) for c in dch) if wch is None else zip(dch, wch)
+    for v, w in pairs:
+        v = np.ascontiguousarray(v, np.float64)
+        if w is not None:
+            if v.shape[0] != np.shape(w)[0]:
+                raise ValueError('`dataset` and `dataset_weights` chunks must be aligned')
+            w = np.ascontiguousarray(w, np.float64)
+        lib().orc_welford_update(ptr(sum_w), ptr(mean), ptr(m2), ptr(delta), ptr(v), ptr(w), v.shape[0], dim)
+    norm = sum_w[0] - ddof
+    if norm <= 0.0:
+        raise ValueError('Empty dataset or degrees of freedom <= 0.')
+    cov = np.empty((dim, dim), np.float64)
+    for j in range(dim):
+        cov[j, j] = m2[j, j] / norm
+        for k in range(j + 1, dim):
+            cov[j, k] = cov[k, j] = m2[j, k] / norm
+    return cov
+
+
+def pca_weights(cov, min_weight, max_weight, m, n, dim, scale_by_variance=False, apply_cdf=False, cdf_gain=1.0):
+    """SOM_PCA._diag_cov_matrix (som_pca.py:116-177) with Covariance.diagonalize (covariance.py:217-250):
+    eigh, flipped to decreasing eigenvalues; the (m, n) grid spans the two leading directions."""
+    import math
+    ev, evec = np.linalg.eigh(cov)
+    ev, evec = np.flip(ev, axis=0), np.flip(evec, axis=1)
+    v0, v1 = evec[:, 0], evec[:, 1]
+    if scale_by_variance:
+        if ev[0] > 0.0:
+            v0 = v0 * np.sqrt(ev[0])
+        if ev[1] > 0.0:
+            v1 = v1 * np.sqrt(ev[1])
+    sigma = np.sqrt(v0 * v0 + v1 * v1)
+    sigma[sigma == 0.0] = 1.0
+    cdf_scale = cdf_gain / (math.sqrt(2.0) * sigma)
+    lx = np.linspace(-1.0, 1.0, m) if apply_cdf else np.linspace(min_weight, max_weight, m)
+    ly = np.linspace(-1.0, 1.0, n) if apply_cdf else np.linspace(min_weight, max_weight, n)
+    W = np.empty((m, n, dim), np.float64)
+    for i in range(m):
+        for j in range(n):
+            for d in range(dim):
+                w = v0[d] * lx[i] + v1[d] * ly[j]
+                W[i, j, d] = (min_weight + (max_weight - min_weight) * 0.5 * (1.0 + math.erf(w * cdf_scale[d]))
+                              if apply_cdf else w)
+    return ev, evec, W
+
+
+def gndm(catalog_activation_map, footprint_activation_map, winners, m, n, dtype=np.float32):
+    """Decontamination_SOM._compute_gndm (decontamination_som.py:600-628): n_gal, n_pix, gnd, gndm."""
+    n_gal = np.sum(catalog_activation_map)
+    n_pix = np.sum(footprint_activation_map)
+    gnd = np.divide(catalog_activation_map, footprint_activation_map,
+                    out=np.full(catalog_activation_map.shape, np.nan, dtype=dtype),
+                    where=footprint_activation_map > 0.0)
+    return n_gal, n_pix, gnd, gnd.reshape(m * n)[winners]
