@@ -48,6 +48,8 @@ def parse():
                     help="skip the extra bf16_rerank run with arithmetic='fp64'")
     ap.add_argument("--stream-chunks", type=int, default=8,
                     help="chunks of the streamed-epoch leg (0 skips it)")
+    ap.add_argument("--no-pipeline", dest="pipeline", action="store_false",
+                    help="skip the covariance / gndm leg")
     ap.add_argument("--serve-seconds", type=float, default=20.0)
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--serve-budget-gb", type=float, default=64.0)
@@ -150,6 +152,87 @@ def streaming_bench(ctx, X, W0, topo, m, sigma0, E, n_chunks, steps=4, warmup=1)
             "overlapped_samples_per_s": round(N / overlap_s, 1),
             "note": "bf16_rerank epoch with X streamed from pageable host memory as generator chunks; "
                     "PCIe-inclusive, not the headline value"}
+
+
+def pipeline_bench(ctx, X, x_dev, W, m, steps=20):
+    """The steps on either side of training, on the same footprint resident in HBM: the weighted
+    covariance of the SOM_PCA initialisation (two passes over X) and the galaxy-number-density maps
+    from the winners (somhip_gndm), timed per call (wall clock, synchronised), with the oracle's
+    single-thread restatements of the reference's CPU loops on a prefix as the CPU baseline."""
+    import ctypes
+    from decontamination_amd import _lib
+    from decontamination_amd.runtime import DeviceArray, Model
+    import oracle
+    from oracle import ptr
+    N, dim = X.shape
+    mn = W.shape[0]
+    rng = np.random.default_rng(1)
+    rho = rng.poisson(3.0, N) / (0.5 + 0.5 * rng.random(N))  # corrected density: counts / coverage
+    r_dev = DeviceArray.from_host(ctx, rho)
+    out = np.empty(1 + dim + dim * dim)
+
+    def cov():
+        _lib.call("somhip_covariance_chunk", ctx.handle, ctypes.c_void_p(x_dev.ptr), 0, ctypes.c_void_p(r_dev.ptr),
+                  N, dim, _lib.vp(out))
+
+    model = Model(ctx, m, m, dim, "hexagonal", np.zeros((mn, 2), np.float32))
+    model.set_weights(W)
+    win = DeviceArray.empty(ctx, (N,), np.int32)
+    model.find_bmus(x_dev.ptr, N, win.ptr, _lib.PREC_BF16_RERANK, _lib.RERANK_REF32)
+    maps = DeviceArray.empty(ctx, (3, mn), np.float32)
+    gm = DeviceArray.empty(ctx, (N,), np.float32)
+
+    def gndm():
+        _lib.call("somhip_gndm", ctx.handle, ctypes.c_void_p(win.ptr), ctypes.c_void_p(r_dev.ptr), N, mn,
+                  ctypes.c_void_p(maps.view_ptr(0)), ctypes.c_void_p(maps.view_ptr(mn)),
+                  ctypes.c_void_p(maps.view_ptr(2 * mn)), ctypes.c_void_p(gm.ptr))
+
+    def timed(fn):
+        fn()
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        ctx.synchronize()
+        return (time.perf_counter() - t0) / steps
+
+    t_cov = timed(cov)
+    t_gndm = timed(gndm)
+    for a in (r_dev, win, maps, gm):
+        a.free()
+    model.close()
+    # CPU baselines: the reference's Welford loop (one thread, covariance.py:30-84) and its two
+    # activation-map passes (_count_bmus_kernel, BMU search per pixel) on prefixes
+    L = oracle.lib()
+    p = 1 << 20
+    xs = np.ascontiguousarray(X[:p], np.float64)
+    sw, mean, m2, dl = np.zeros(1), np.zeros(dim), np.zeros((dim, dim)), np.zeros(dim)
+    t0 = time.perf_counter()
+    L.orc_welford_update(ptr(sw), ptr(mean), ptr(m2), ptr(dl), ptr(xs), ptr(rho[:p]), p, dim)
+    cpu_cov = p / (time.perf_counter() - t0)
+    q = 1 << 13
+    xq = np.ascontiguousarray(X[:q])
+    res = np.zeros(mn)
+    t0 = time.perf_counter()
+    L.orc_count_bmus(0, ptr(W), mn, dim, ptr(xq), ptr(rho[:q]), q, ptr(res))
+    L.orc_count_bmus(0, ptr(W), mn, dim, ptr(xq), None, q, ptr(res))
+    cpu_gndm = q / (time.perf_counter() - t0)
+    algo_cov = N * (4 * dim + 8)
+    algo_gndm = N * (4 + 8 + 4)
+    return {
+        "covariance": {"ms": round(1e3 * t_cov, 4), "rows_per_s": round(N / t_cov, 1),
+                       "algorithmic_GB_per_s": round(algo_cov / t_cov / 1e9, 1),
+                       "note": f"weighted covariance of the {N}x{dim} footprint (SOM_PCA init), 2 passes over X, "
+                               "wall clock per call incl. the 2 KB result copy",
+                       "cpu_baseline": {"value": round(cpu_cov, 1), "unit": "rows/s", "cores": 1, "kind": "port",
+                                        "sample": f"oracle Welford loop on the first {p} rows"}},
+        "gndm": {"ms": round(1e3 * t_gndm, 4), "pixels_per_s": round(N / t_gndm, 1),
+                 "algorithmic_GB_per_s": round(algo_gndm / t_gndm / 1e9, 1),
+                 "note": "catalog + footprint activation maps from the winners, gnd and gndm = gnd[winners]; the "
+                         "reference runs two more BMU searches over the footprint for the maps",
+                 "cpu_baseline": {"value": round(cpu_gndm, 1), "unit": "pixels/s", "cores": 1, "kind": "port",
+                                  "sample": f"oracle _count_bmus_kernel loop (two maps) on the first {q} pixels"}},
+    }
 
 
 TRAFFIC_PROFILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
@@ -468,6 +551,9 @@ def main():
         st["resident_ms_per_epoch"] = round(head["ms_per_step"], 4)
         log(f"streaming: {json.dumps(st)}")
         out["streaming"] = st
+    if args.pipeline and "bf16_rerank" in results:
+        out["pipeline"] = pipeline_bench(ctx, X, x_dev, head["W"], m)
+        log(f"pipeline: {json.dumps(out['pipeline'])}")
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(X, W0, topo, sigma0, E)
     out["serving"] = None
