@@ -50,6 +50,8 @@ def parse():
                     help="chunks of the streamed-epoch leg (0 skips it)")
     ap.add_argument("--no-pipeline", dest="pipeline", action="store_false",
                     help="skip the covariance / gndm leg")
+    ap.add_argument("--c0", action="store_true",
+                    help="also time the 10x10/D8/100k CPU-runnable configuration (CPU oracle and GPU)")
     ap.add_argument("--serve-seconds", type=float, default=20.0)
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--serve-budget-gb", type=float, default=64.0)
@@ -291,9 +293,75 @@ def cpu_baseline(X, W0, topo, sigma0, E, target_s=12.0):
     p2 = max(1024, min(p2, 1 << 17))
     t2 = run(p2)
     return {"value": round(p2 / t2, 2), "unit": "samples/s", "cores": threads, "kind": "port",
+            "cpu_model": cpu_model(),
             "sample": f"first {p2} samples of the same 64x64/D16 workload, one epoch (reference loop nest: "
                       f"parallel over samples, FP32 atomic scatter to all {mn} neurons, plus the errors pass), "
                       f"{t2:.2f} s; extrapolated per sample"}
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def c0_bench():
+    """The reference's CPU-runnable configuration in full: 10x10 square map, D=8, 100,000 samples of a
+    seeded 5-Gaussian mixture, 10 epochs from sigma0=5: the oracle's OpenMP restatement of the
+    reference CPU epoch (median of 3 runs) beside the GPU path on the same inputs."""
+    import decontamination_amd as dam
+    import oracle
+    from oracle import ptr
+    from oracle import som_ref as R
+    rng = np.random.default_rng(0)
+    means = rng.uniform(-3, 3, (5, 8))
+    labels = rng.integers(0, 5, 100_000)
+    X = (means[labels] + 0.5 * rng.standard_normal((100_000, 8))).astype(np.float32)
+    W0 = rng.uniform(-1, 1, (100, 8)).astype(np.float32)
+    m, E, sigma0 = 10, 10, 5.0
+    topo = R.topography(m, m, "square")
+    L = oracle.lib()
+
+    def cpu_run():
+        W = W0.copy()
+        t0 = time.perf_counter()
+        for e in range(E):
+            num = np.zeros((100, 8), np.float32)
+            den = np.zeros(100, np.float32)
+            L.orc_train_epoch_accumulate_omp(ptr(num), ptr(den), ptr(W), ptr(topo), ptr(X), None, len(X), e, E,
+                                             np.float32(sigma0), 100, 8)
+            Wn = np.empty((100, 8), np.float32)
+            L.orc_divide(0, ptr(num), ptr(den), 100, 8, ptr(Wn))
+            err = np.zeros(3)
+            L.orc_compute_errors_omp(ptr(Wn), ptr(topo), 100, 8, ptr(X), None, len(X), 2.0, ptr(err))
+            W = Wn
+        return time.perf_counter() - t0, W
+
+    runs = [cpu_run() for _ in range(3)]
+    cpu_s = sorted(r[0] for r in runs)[1]
+
+    def gpu_run():
+        som = dam.SOM_Batch(m, m, 8, topology="square", sigma=sigma0)
+        som._weights[...] = W0
+        t0 = time.perf_counter()
+        som.train(X, n_epochs=E, use_best_epoch=False)
+        return time.perf_counter() - t0, som.weights
+
+    gpu_run()
+    gruns = [gpu_run() for _ in range(3)]
+    gpu_s = sorted(r[0] for r in gruns)[1]
+    dW = float(np.abs(gruns[0][1] - runs[0][1]).max())
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"workload": "10x10 square, D=8, N=100000 (5-Gaussian mixture, seed 0), 10 epochs, sigma0=5",
+            "cpu": {"s_total": round(cpu_s, 4), "samples_per_s": round(E * len(X) / cpu_s, 1), "cores": threads,
+                    "cpu_model": cpu_model(), "kind": "port", "runs": 3, "stat": "median"},
+            "gpu": {"s_total": round(gpu_s, 4), "samples_per_s": round(E * len(X) / gpu_s, 1),
+                    "note": "SOM_Batch.train wall time incl. upload, per-epoch errors and history copy"},
+            "max_abs_weight_diff_gpu_vs_cpu": dW}
 
 
 def accuracy(ctx, res, X, topo, m, prec, arith, E, sigma0, n_sub=1 << 20):
@@ -556,6 +624,9 @@ def main():
         log(f"pipeline: {json.dumps(out['pipeline'])}")
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(X, W0, topo, sigma0, E)
+    if args.c0:
+        out["c0"] = c0_bench()
+        log(f"c0: {json.dumps(out['c0'])}")
     out["serving"] = None
     if args.serve_seconds > 0:
         del x_dev
