@@ -39,6 +39,9 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--nside", type=int, default=1024)
     ap.add_argument("--map", type=int, default=64)
+    ap.add_argument("--lognormal", type=int, default=8, help="lognormal feature columns")
+    ap.add_argument("--gaussian", type=int, default=6, help="Gaussian feature columns (+2 angular)")
+    ap.add_argument("--acc-subsample", type=int, default=1 << 20)
     ap.add_argument("--epochs-schedule", type=int, default=20)
     ap.add_argument("--arithmetic", default="ref32")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -525,7 +528,7 @@ def main():
     E = args.epochs_schedule
     sigma0 = m / 2.0
     t = time.perf_counter()
-    X = synth.systematics_dataset(args.nside, 8, 6, seed=0)
+    X = synth.systematics_dataset(args.nside, args.lognormal, args.gaussian, seed=0)
     N, dim = X.shape
     log(f"data {N}x{dim} generated in {time.perf_counter() - t:.1f}s")
     som = dam.SOM_Batch(m, m, dim)
@@ -569,7 +572,7 @@ def main():
         log(f"bf16_rerank/fp64: {extra['ms_per_step']:.3f} ms/epoch")
 
     flops = 2.0 * N * m * m * dim
-    default_workload = m == 64 and args.nside == 1024
+    default_workload = m == 64 and args.nside == 1024 and dim == 16
     head = results.get("bf16_rerank") or next(iter(results.values()))
     head_prec = "bf16_rerank" if "bf16_rerank" in results else next(iter(results))
     out = {
@@ -604,14 +607,14 @@ def main():
     if not args.no_accuracy:
         acc = {}
         for prec, r in results.items():
-            acc[prec] = accuracy(ctx, r, X, topo, m, prec, args.arithmetic, E, sigma0)
+            acc[prec] = accuracy(ctx, r, X, topo, m, prec, args.arithmetic, E, sigma0, args.acc_subsample)
             log(f"accuracy {prec}: {json.dumps(acc[prec])}")
         out["accuracy"] = acc
     if extra is not None:
         out["bf16_rerank_fp64_arithmetic"] = {"value": round(extra["samples_per_s"], 1),
                                               "ms_per_step": round(extra["ms_per_step"], 4)}
         if not args.no_accuracy:
-            a64 = accuracy(ctx, extra, X, topo, m, "bf16_rerank", "fp64", E, sigma0)
+            a64 = accuracy(ctx, extra, X, topo, m, "bf16_rerank", "fp64", E, sigma0, args.acc_subsample)
             log(f"accuracy bf16_rerank/fp64: {json.dumps(a64)}")
             out["bf16_rerank_fp64_arithmetic"]["accuracy"] = a64
     if args.stream_chunks > 0 and "bf16_rerank" in results:

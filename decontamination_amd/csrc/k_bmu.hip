@@ -26,7 +26,6 @@ namespace {
 
 constexpr int WAVE = 64;
 constexpr int BMU_WAVES = 4;                 // waves per workgroup of the MFMA kernels
-constexpr int CAPL = 20;                     // recorded candidate neurons per lane (BF16 window)
 constexpr float U32 = 5.9604644775390625e-8f;  // 2^-24
 
 // Each lane needs its partner (lane ^ 32) value: both hold the same sample column.
@@ -309,12 +308,24 @@ constexpr int CHUNK_KB = 8;             // (32-neuron block) x (16-wide K slice)
 constexpr int FRAG_UNITS = CHUNK_KB * 2 * 64;  // 16-byte units of A fragments per chunk
 static_assert(NW_BF16 * G_BF16 * 32 == SOM_TILE_ROWS, "tile rows must match the BF16 workgroup");
 
+// Candidate lists. D = 16: 20 per lane; a lane past that sends its sample to the fallback.
+// D = 32 (more neurons closer than the window, on large smooth maps): 16 per lane plus a pool of 128
+// shared by the 32-sample group for lanes past that, in the same LDS; only a full pool sends a sample
+// to the fallback. The pool's spill test costs a few percent at D = 16, where lists rarely fill.
+template <int KC>
+struct Bf16Cand {
+    static constexpr int CAPL = KC == 1 ? 20 : 16;  // own list per lane
+    static constexpr int POOL = KC == 1 ? 0 : 128;  // shared entries per group
+};
+
 template <int KC>
 struct Bf16Smem {
     static constexpr int CB = CHUNK_KB / KC;  // blocks per chunk
+    static constexpr int CAPL = Bf16Cand<KC>::CAPL;
     uint4 frag[2][FRAG_UNITS];          // [buf][cb][kc][hi,lo][lane] 16 B
     float norms[2][CB * 32];            // [buf][cb][row]
     unsigned short cand[NW_BF16][G_BF16][CAPL + 1][WAVE];  // last row: writes past a full list
+    unsigned int pool[NW_BF16][G_BF16][Bf16Cand<KC>::POOL > 0 ? Bf16Cand<KC>::POOL : 1];  // (lane << 16) | slot
 };
 
 template <int DP, int G, int MODE, bool NEED_C2>
@@ -326,6 +337,8 @@ __device__ __forceinline__ void bmu_bf16_body(const BmuModelDev &Md, const float
 {
     constexpr int KC = DP / 16;
     constexpr int CB = Bf16Smem<KC>::CB;
+    constexpr int CAPL = Bf16Cand<KC>::CAPL;
+    constexpr int POOL = Bf16Cand<KC>::POOL;
     constexpr int NORM_UNITS = CB * 8;   // 16-byte units of norms per chunk
     constexpr int NT = NW_BF16 * WAVE;   // threads per workgroup
     constexpr int FU = FRAG_UNITS / NT;  // fragment units per thread
@@ -498,14 +511,20 @@ __device__ __forceinline__ void bmu_bf16_body(const BmuModelDev &Md, const float
         m2[g] = __builtin_amdgcn_fmed3f(m1[g], m2[g], t);
         m1[g] = fminf(m1[g], t);
     };
-    int cnt[G];
+    int cnt[G], pool_n[G];  // pool_n: wave-uniform fill of the group's shared pool
+    bool lost[G];           // a candidate of this lane found both its list and the pool full (D = 32)
 #pragma unroll
-    for (int g = 0; g < G; g++) cnt[g] = 0;
+    for (int g = 0; g < G; g++) {
+        cnt[g] = 0;
+        pool_n[g] = 0;
+        lost[g] = false;
+    }
     // Appends every row of this lane's half-block with D <= T; the pre-check skips the per-value
     // compares when no lane of the wave has a hit in the block.
     // The 16 row compares are issued back to back as wave masks; rows without any hit cost one scalar
-    // test. A full list keeps counting (overflow sends the sample to the fallback) and writes to the
-    // spare last row.
+    // test. Past CAPL a lane's candidates go to its group's shared pool when there is one (slots handed
+    // out in lane order from the wave's ballot), else the lane is lost to the fallback. Lanes without a
+    // hit, and candidates past the list, write their own-list store to the spare last row.
     auto record = [&](int blk, int g, const f32x16 &v, float t) {
         if (__builtin_amdgcn_ballot_w64(t <= T[g]) != 0ull) {
             const int sbase_blk = blk * 32 + 4 * khalf;
@@ -516,7 +535,19 @@ __device__ __forceinline__ void bmu_bf16_body(const BmuModelDev &Md, const float
             for (int r = 0; r < 16; r++) {
                 if (__builtin_expect(bm[r] != 0ull, 0)) {  // uniform, rare: out of line
                     const bool hit = v[r] <= T[g];  // lanes without a hit write to the spare row
-                    cw[g][hit ? min(cnt[g], CAPL) : CAPL][lane] = (unsigned short)(sbase_blk + 8 * (r >> 2) + (r & 3));
+                    const unsigned short slot = (unsigned short)(sbase_blk + 8 * (r >> 2) + (r & 3));
+                    cw[g][hit ? min(cnt[g], CAPL) : CAPL][lane] = slot;
+                    if constexpr (POOL > 0) {
+                        const bool spill = hit && cnt[g] >= CAPL;
+                        const unsigned long long sp = __builtin_amdgcn_ballot_w64(spill);
+                        if (__builtin_expect(sp != 0ull, 0)) {  // uniform, rarer still
+                            const int off = pool_n[g] + (int)__builtin_amdgcn_mbcnt_hi(
+                                                (unsigned)(sp >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)sp, 0u));
+                            if (spill && off < POOL) sm.pool[wave][g][off] = ((unsigned)lane << 16) | slot;
+                            lost[g] = lost[g] || (spill && off >= POOL);
+                            pool_n[g] += __builtin_popcountll(sp);
+                        }
+                    }
                     cnt[g] += hit ? 1 : 0;
                 }
             }
@@ -560,8 +591,8 @@ __device__ __forceinline__ void bmu_bf16_body(const BmuModelDev &Md, const float
         const bool valid_s = s < n;
         Top2 tp;
         tp.init();
-        const int mycnt = cnt[g];
-        const bool ovf = mycnt > CAPL;
+        const int mycnt = min(cnt[g], CAPL);
+        const bool ovf = POOL > 0 ? lost[g] : cnt[g] > CAPL;
         if (valid_s && !ovf) {
             float xr[DP];
 #pragma unroll
@@ -588,6 +619,23 @@ __device__ __forceinline__ void bmu_bf16_body(const BmuModelDev &Md, const float
                 const int j = jn;
                 if (e + 1 < mycnt) load_row(e + 1, wn, jn);
                 if (j >= 0) tp.push(sqd_mode<MODE>(wr, xr, DP), j);  // zero-padded dims add exact zeros
+            }
+            // the group's spilled candidates: each entry is reranked by the lane that recorded it
+            const int np = POOL > 0 ? min(pool_n[g], POOL) : 0;
+            for (int e = 0; e < np; e++) {
+                const unsigned int ent = sm.pool[wave][g][e];
+                if ((int)(ent >> 16) == lane) {
+                    float wr[DP];
+                    const int slot = (int)(ent & 0xFFFFu);
+                    const int j = perm[slot];
+                    const f32x4 *row = (const f32x4 *)(wslot + (int64_t)slot * DP);
+#pragma unroll
+                    for (int q = 0; q < DP / 4; q++) {
+                        const f32x4 t = row[q];
+                        wr[4 * q] = t[0]; wr[4 * q + 1] = t[1]; wr[4 * q + 2] = t[2]; wr[4 * q + 3] = t[3];
+                    }
+                    if (j >= 0) tp.push(sqd_mode<MODE>(wr, xr, DP), j);
+                }
             }
         }
         // merge with the partner lane
