@@ -3,7 +3,9 @@
 
 #include "common.hpp"
 
-constexpr size_t SCATTER_TABLE_BUDGET = (size_t)192 << 20;  // bytes of per-workgroup partial tables
+// bytes of per-workgroup partial tables: enough for one table per CU up to 100x100 neurons x 33 values
+// (2.6 MB each), so large maps keep every CU busy; a small share of the 288 GB of HBM
+constexpr size_t SCATTER_TABLE_BUDGET = (size_t)1 << 30;
 
 int somhip_launch_absmax(hipStream_t st, const float *x, int dim, int dpad, const double *rho, int rho_f32, int64_t n,
                          unsigned int *bounds, int num_cus);
