@@ -138,3 +138,32 @@ def test_streamed_training_matches_oracle():
     assert np.allclose(som.weights, Wr, rtol=1e-6, atol=1e-7)
     assert np.allclose(som.quantization_errors, qe, rtol=1e-6)
     assert np.allclose(som.topographic_errors, te, rtol=1e-9, atol=0)
+
+
+def test_large_chunk_wraps_the_staging_ring_and_pinned_chunks():
+    """A 160 MB chunk takes five 32 MB pinned pieces from a ring of four (the ring wraps inside one
+    put, so a piece is refilled only after its DMA); chunks in pinned memory skip the staging. Both give
+    the BMUs of the resident path, which are pinned against the oracle on a sample of rows."""
+    from decontamination_amd.serving import pinned_empty
+    m, n, dim, N = 24, 24, 16, 2_600_000
+    print("large chunk through the staging ring", flush=True)
+    rng = np.random.default_rng(71)
+    X = rng.standard_normal((N, dim), dtype=np.float32)
+    som = dam.SOM_Batch(m, n, dim)
+    som._weights[...] = rng.standard_normal((m * n, dim)).astype(np.float32)
+    want = som.get_winners(X)  # one pageable chunk of 166 MB: staged, ring wraps
+    ctx = get_context(0)
+    xd = DeviceArray.from_host(ctx, X)
+    out = DeviceArray.empty(ctx, (N,), np.int32)
+    som._device_model().find_bmus(xd.ptr, N, out.ptr, _lib.PREC_BF16_RERANK, _lib.RERANK_REF32)
+    assert np.array_equal(want, out.copy_to_host())
+    idx = rng.choice(N, 20_000, replace=False)
+    assert np.array_equal(want[idx], R.find_bmus(som.weights, X[idx], REF32))
+    # the same rows as pinned chunks (copied directly) and as pageable chunks, through a generator
+    pinned = [pinned_empty((s, dim), np.float32) for s in (1_000_000, 1_600_000)]
+    pinned[0][...] = X[:1_000_000]
+    pinned[1][...] = X[1_000_000:]
+    act_pinned = som.get_activation_map(lambda: lambda: iter(pinned))
+    act_pageable = som.get_activation_map(lambda: lambda: iter([X[:1_000_000], X[1_000_000:]]))
+    want_act = np.bincount(want, minlength=m * n).reshape(m, n).astype(np.float32)
+    assert np.array_equal(act_pinned, want_act) and np.array_equal(act_pageable, want_act)
