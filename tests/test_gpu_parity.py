@@ -304,3 +304,33 @@ def test_empty_and_invalid_inputs():
     assert np.array_equal(som.get_activation_map(np.zeros((0, 4), np.float32)), np.zeros((3, 3), np.float32))
     with pytest.raises(ValueError):
         som.get_winners(np.zeros((5, 3), np.float32))
+
+
+def test_cpu_configuration_in_full():
+    """The reference's CPU-runnable configuration in full: 10x10 square map, D=8, 100,000 samples of a
+    seeded 5-Gaussian mixture (means uniform in [-3, 3], sigma 0.5), weights uniform in [-1, 1], 10
+    epochs from sigma0=5. With FP64 arithmetic the GPU path equals the FP64 oracle to about an FP32 ulp
+    (no FP32 accumulator on either side). With the reference arithmetic its exact sums land closer to
+    the FP64 oracle than the reference's own FP32-accumulator restatement does (max |dW| 2e-3 here)."""
+    rng = np.random.default_rng(0)
+    means = rng.uniform(-3, 3, (5, 8))
+    labels = rng.integers(0, 5, 100_000)
+    X = (means[labels] + 0.5 * rng.standard_normal((100_000, 8))).astype(np.float32)
+    W0 = rng.uniform(-1, 1, (100, 8)).astype(np.float32)
+    kw = dict(n_epochs=10, sigma=5.0, topology="square", use_best_epoch=False)
+    W64, _, qe64, te64 = R.train(W0, 10, 10, X, mode=FP64, **kw)
+    W32, _, qe32, _ = R.train(W0, 10, 10, X, mode=REF32, **kw)
+    soms = {}
+    for arith in ("fp64", "ref32"):
+        som = dam.SOM_Batch(10, 10, 8, topology="square", sigma=5.0, arithmetic=arith)
+        som._weights[...] = W0
+        som.train(X, n_epochs=10, use_best_epoch=False)
+        soms[arith] = som
+    g = soms["fp64"]
+    assert np.allclose(g.weights, W64, rtol=1e-6, atol=1e-6)
+    assert np.allclose(g.quantization_errors, qe64, rtol=1e-7)
+    assert np.allclose(g.topographic_errors, te64, rtol=1e-12, atol=0)
+    r = soms["ref32"]
+    assert np.abs(r.weights - W64).max() < np.abs(W32 - W64).max()
+    assert np.allclose(r.quantization_errors, qe32, rtol=1e-3)
+    assert np.array_equal(r.get_winners(X), R.find_bmus(r.weights, X, REF32))
