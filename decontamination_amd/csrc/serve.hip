@@ -3,8 +3,12 @@
 //
 //   submit()  -> pending queue (FIFO)
 //   admission -> a request is admitted when its working memory fits in the free part of the budget
-//                (one device slab, first-fit); its rows are uploaded on the copy stream
-//   batching  -> every iteration, the tiles (512 rows) of all uploaded requests are shared out
+//                (one device slab, first-fit)
+//   upload    -> rows move to the device on the copy stream in pieces of whole tiles (at most 16 MB),
+//                at most 3 pieces in flight, taken round-robin from the requests being uploaded: a
+//                large request does not hold the small ones behind its whole upload, and its first
+//                tiles are computed while its later pieces are still on the way
+//   batching  -> every iteration, the uploaded tiles (512 rows) of all requests are shared out
 //                round-robin into one launch per dimension class; the launch is a hipGraph captured
 //                per (dimension, tile-count bucket) with a fixed tile-table buffer, so a replay is a
 //                table upload plus one graph launch
@@ -40,6 +44,10 @@ constexpr int MAX_TILES = 4096;
 constexpr int STAGING_SLOTS = 8;
 constexpr size_t ALIGN = 256;
 constexpr size_t ACT_TABLE_BUDGET = (size_t)32 << 20;
+// Uploads move in pieces of at most UPLOAD_PIECE bytes (whole tiles), at most MAX_INFLIGHT of them on
+// the copy stream at a time, taken round-robin from the requests being uploaded.
+constexpr size_t UPLOAD_PIECE = (size_t)16 << 20;
+constexpr int MAX_INFLIGHT = 3;
 
 inline size_t align_up(size_t v) { return (v + ALIGN - 1) / ALIGN * ALIGN; }
 
@@ -111,8 +119,11 @@ struct Request {
     unsigned int *bounds = nullptr;
     int *shifts = nullptr;
     int nwg = 1;
-    int64_t next_row = 0;
-    hipEvent_t ev_up = nullptr, ev_done = nullptr;
+    int64_t next_row = 0;     // first row not yet handed to a launch
+    int64_t up_rows = 0;      // rows whose copy is queued on the copy stream
+    int64_t ready_rows = 0;   // rows on the device (their pieces seen complete)
+    std::deque<std::pair<int64_t, hipEvent_t>> pieces;  // queued copies in order: (end row, event)
+    hipEvent_t ev_done = nullptr;
     Clock::time_point t_submit, t_done;
 };
 
@@ -139,6 +150,9 @@ struct somhip_server {
     BmuTile *staging = nullptr;  // pinned host, STAGING_SLOTS * MAX_TILES
     hipEvent_t staging_ev[STAGING_SLOTS] = {};
     int staging_next = 0;
+    std::vector<hipEvent_t> ev_pool;  // completed piece events for reuse
+    int inflight = 0;                 // pieces queued and not yet seen complete
+    size_t up_cursor = 0;             // round-robin position among the requests being uploaded
 
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
@@ -154,6 +168,8 @@ struct somhip_server {
     std::atomic<int64_t> completed{0}, launches{0}, tiles_launched{0};
 
     int build_graph(int dclass, int b);
+    int queue_piece(Request &R);
+    bool poll_pieces(Request &R, hipError_t *err);
     int launch_tiles(int dclass, const std::vector<BmuTile> &tl);
     int finish(Request &R);
     void loop();
@@ -243,6 +259,52 @@ int somhip_server::finish(Request &R)
     return SOMHIP_OK;
 }
 
+// Queues the next piece of R's rows (and weights) on the copy stream.
+int somhip_server::queue_piece(Request &R)
+{
+    somhip_model *M = models[R.model];
+    const size_t row_bytes = sizeof(float) * M->dim + (R.rho ? sizeof(double) : 0);
+    const int64_t tiles = std::max<int64_t>(1, (int64_t)(UPLOAD_PIECE / row_bytes) / SOM_TILE_ROWS);
+    const int64_t r0 = R.up_rows, r1 = std::min(R.n, r0 + tiles * SOM_TILE_ROWS);
+    hipEvent_t e = nullptr;
+    if (!ev_pool.empty()) {
+        e = ev_pool.back();
+        ev_pool.pop_back();
+    } else {
+        SOM_HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    }
+    R.pieces.emplace_back(r1, e);  // owned by the request from here, even if a copy below fails
+    SOM_HIP_TRY(hipMemcpyAsync(R.x + r0 * M->dim, R.x_host + r0 * M->dim, sizeof(float) * (r1 - r0) * M->dim,
+                               hipMemcpyHostToDevice, copy));
+    if (R.rho)
+        SOM_HIP_TRY(hipMemcpyAsync(R.rho + r0, R.rho_host + r0, sizeof(double) * (r1 - r0), hipMemcpyHostToDevice,
+                                   copy));
+    SOM_HIP_TRY(hipEventRecord(e, copy));
+    R.up_rows = r1;
+    inflight++;
+    return SOMHIP_OK;
+}
+
+// Advances R.ready_rows over its completed pieces; returns whether any completed.
+bool somhip_server::poll_pieces(Request &R, hipError_t *err)
+{
+    bool any = false;
+    while (!R.pieces.empty()) {
+        const hipError_t q = hipEventQuery(R.pieces.front().second);
+        if (q == hipErrorNotReady) break;
+        if (q != hipSuccess) {
+            *err = q;
+            return any;
+        }
+        R.ready_rows = R.pieces.front().first;
+        ev_pool.push_back(R.pieces.front().second);
+        R.pieces.pop_front();
+        inflight--;
+        any = true;
+    }
+    return any;
+}
+
 void somhip_server::loop()
 {
     if (hipSetDevice(ctx->device) != hipSuccess) {
@@ -274,51 +336,58 @@ void somhip_server::loop()
                     R->shifts = (int *)take(512);
                     R->table = (unsigned long long *)take(sizeof(unsigned long long) * (size_t)R->nwg * M->mn);
                 }
+                R->state = R->n > 0 ? UPLOADING : READY;
                 active.push_back(R);
-                g.unlock();
-                hipError_t e = hipSuccess;
-                if (R->n > 0)
-                    e = hipMemcpyAsync(R->x, R->x_host, sizeof(float) * R->n * M->dim, hipMemcpyHostToDevice, copy);
-                if (e == hipSuccess && R->rho)
-                    e = hipMemcpyAsync(R->rho, R->rho_host, sizeof(double) * R->n, hipMemcpyHostToDevice, copy);
-                if (e == hipSuccess) e = hipEventRecord(R->ev_up, copy);
-                g.lock();
-                if (e != hipSuccess) {
-                    g.unlock();
-                    fail(std::string("upload: ") + hipGetErrorString(e));
-                    return;
-                }
-                R->state = UPLOADING;
                 progress = true;
             }
         }
-        // uploads that completed
+        // pieces that completed
         for (auto &R : active) {
-            if (R->state == UPLOADING) {
-                hipError_t q = hipEventQuery(R->ev_up);
-                if (q == hipSuccess) {
-                    R->state = READY;
-                    progress = true;
-                } else if (q != hipErrorNotReady) {
-                    fail(std::string("upload event: ") + hipGetErrorString(q));
-                    return;
+            hipError_t q = hipSuccess;
+            if (poll_pieces(*R, &q)) progress = true;
+            if (q != hipSuccess) {
+                fail(std::string("upload event: ") + hipGetErrorString(q));
+                return;
+            }
+            if (R->state == UPLOADING && R->ready_rows == R->n) R->state = READY;
+        }
+        // more pieces, round-robin over the requests with rows left to upload
+        while (inflight < MAX_INFLIGHT && !active.empty()) {
+            Request *pick = nullptr;
+            for (size_t k = 0; k < active.size(); k++) {
+                Request *R = active[(up_cursor + k) % active.size()].get();
+                if (R->state == UPLOADING && R->up_rows < R->n) {
+                    pick = R;
+                    up_cursor = (up_cursor + k + 1) % active.size();
+                    break;
                 }
             }
+            if (!pick) break;
+            if (queue_piece(*pick)) {
+                fail(std::string("upload: ") + somhip_last_error());
+                return;
+            }
+            progress = true;
         }
         // batching: share tiles round-robin among the ready requests of each dimension class
         for (int dc = 0; dc < 2; dc++) {
-            std::vector<Request *> ready;
-            for (auto &R : active)
-                if (R->state == READY && dclass_of(model_dpad[R->model]) == dc) ready.push_back(R.get());
-            if (ready.empty()) continue;
+            // requests with uploaded rows not yet launched (pieces are whole tiles, so a tile below
+            // ready_rows is on the device), and finished requests (every row launched)
+            std::vector<Request *> ready, done_rows;
+            for (auto &R : active) {
+                if (dclass_of(model_dpad[R->model]) != dc) continue;
+                if ((R->state == UPLOADING || R->state == READY) && R->next_row < R->ready_rows) ready.push_back(R.get());
+                else if (R->state == READY && R->next_row >= R->n) done_rows.push_back(R.get());
+            }
+            if (ready.empty() && done_rows.empty()) continue;
             std::vector<BmuTile> tl;
             tl.reserve(MAX_TILES);
             std::vector<Request *> finished;
-            const int share = std::max(16, MAX_TILES / (int)ready.size());
+            const int share = ready.empty() ? 0 : std::max(16, MAX_TILES / (int)ready.size());
             for (int pass = 0; pass < 2 && (int)tl.size() < MAX_TILES; pass++) {
                 for (Request *R : ready) {
                     int quota = pass == 0 ? share : MAX_TILES;
-                    while (R->next_row < R->n && quota > 0 && (int)tl.size() < MAX_TILES) {
+                    while (R->next_row < R->ready_rows && quota > 0 && (int)tl.size() < MAX_TILES) {
                         BmuTile t;
                         t.x = R->x;
                         t.n = R->n;
@@ -335,7 +404,8 @@ void somhip_server::loop()
                 }
             }
             for (Request *R : ready)
-                if (R->next_row >= R->n) finished.push_back(R);
+                if (R->state == READY && R->next_row >= R->n) finished.push_back(R);
+            for (Request *R : done_rows) finished.push_back(R);
             if (!tl.empty()) {
                 int rc = launch_tiles(dc, tl);
                 if (rc) {
@@ -446,9 +516,10 @@ int somhip_server_destroy(somhip_server *S)
     (void)hipStreamSynchronize(S->compute);
     (void)hipStreamSynchronize(S->copy);
     for (auto &kv : S->all) {
-        if (kv.second->ev_up) (void)hipEventDestroy(kv.second->ev_up);
+        for (auto &pc : kv.second->pieces) (void)hipEventDestroy(pc.second);
         if (kv.second->ev_done) (void)hipEventDestroy(kv.second->ev_done);
     }
+    for (auto e : S->ev_pool) (void)hipEventDestroy(e);
     for (int dc = 0; dc < 2; dc++)
         for (int b = 0; b < NBUCKETS; b++) {
             GraphSlot &G = S->graphs[dc][b];
@@ -550,7 +621,6 @@ int somhip_server_submit(somhip_server *S, int model_id, int kind, const float *
         somhip_set_error("request exceeds the serving memory budget");
         return SOMHIP_ERR_ARG;
     }
-    SOM_HIP_TRY(hipEventCreateWithFlags(&R->ev_up, hipEventDisableTiming));
     SOM_HIP_TRY(hipEventCreateWithFlags(&R->ev_done, hipEventDisableTiming));
     {
         std::lock_guard<std::mutex> g(S->mu);
@@ -583,9 +653,8 @@ int somhip_server_wait(somhip_server *S, int64_t ticket, double timeout_s, int *
     if (*done) {
         if (latency_ms)
             *latency_ms = std::chrono::duration<double, std::milli>(R->t_done - R->t_submit).count();
-        if (R->ev_up) (void)hipEventDestroy(R->ev_up);
         if (R->ev_done) (void)hipEventDestroy(R->ev_done);
-        R->ev_up = R->ev_done = nullptr;
+        R->ev_done = nullptr;
         S->all.erase(it);
     }
     return SOMHIP_OK;

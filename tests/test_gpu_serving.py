@@ -36,6 +36,14 @@ def make_requests(soms):
             rho = rng.random(rows) * 2.0
             rho[::9] = 0.0
         reqs.append((mi, kind, x, rho))
+    # uploads of several 16 MB pieces (row counts not multiples of the 512-row tile): their first tiles
+    # are computed while the later pieces are still on the way
+    for mi, kind, rows in ((1, "activation", 600_001), (0, "winners", 700_003)):
+        d = soms[mi].dim
+        x = pinned_empty((rows, d), np.float32)
+        x[...] = rng.standard_normal((rows, d)).astype(np.float32)
+        rho = rng.random(rows) * 2.0 if kind == "activation" else None
+        reqs.append((mi, kind, x, rho))
     return reqs
 
 
