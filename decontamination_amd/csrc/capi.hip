@@ -347,6 +347,19 @@ int somhip_model_create(somhip_ctx *C, int m, int n, int dim, int topology, cons
     M->dpad = (dim + 15) / 16 * 16;
     M->topology = topology;
     M->nblk = (M->mn + 31) / 32;
+    // the reference's grid (som_abstract.py:81-111): neuron i*n + j at (i, j), or (i + 0.5 (j % 2),
+    // j sqrt(3)/2) in FP32 for hexagonal; only then does the neighbourhood factor into rows and columns
+    {
+        bool sep = m <= 2048 && n <= 2048;
+        for (int i = 0; sep && i < m; i++)
+            for (int j = 0; sep && j < n; j++) {
+                const float ex = topology == SOMHIP_TOPOLOGY_SQUARE ? (float)i : (float)i + 0.5f * (float)(j % 2);
+                const float ey = topology == SOMHIP_TOPOLOGY_SQUARE ? (float)j : (float)(j * 0.8660254037844386);
+                const float *t = topo_host + 2 * ((size_t)i * n + j);
+                sep = t[0] == ex && t[1] == ey;
+            }
+        M->separable = sep;
+    }
     PermLayout layout = make_perm(m, n);
     M->pblk = layout.pblk;
     M->pblk_a = layout.pblk_a;
@@ -609,9 +622,18 @@ int somhip_batch_epoch_update(somhip_accum *A, int64_t cur_epoch, int64_t n_epoc
     // sigma = sigma0 * asymptotic_decay(e, E) >= FLOAT_EPS (som_batch.py:107-110, algo/__init__.py:64-67)
     double sigma = (double)sigma0 * (1.0 / (1.0 + 2.0 * (double)cur_epoch / (double)n_epochs));
     if (sigma < SOM_FLOAT_EPS) sigma = SOM_FLOAT_EPS;
+    int rc;
+    if (M->separable && somhip_update_separable_fits(M->m, M->n, M->dim)) {
+        if ((rc = A->upd.ensure(sizeof(double) * 2 * (size_t)M->mn * (M->dim + 1)))) return rc;
+        TimerScope ts(M->ctx, TK_UPDATE);
+        rc = somhip_launch_update_separable(M->ctx->stream, A->acc, M->topo, M->m, M->n,
+                                            M->topology == SOMHIP_TOPOLOGY_HEXAGONAL ? 1 : 0, M->dim, sigma,
+                                            (double *)A->upd.p, M->w, history_out);
+        M->prepared = false;
+        return rc;
+    }
     const int nsplit = somhip_update_nsplit(M->mn);
-    int rc = A->upd.ensure(sizeof(double) * (size_t)nsplit * M->mn * (M->dim + 1));
-    if (rc) return rc;
+    if ((rc = A->upd.ensure(sizeof(double) * (size_t)nsplit * M->mn * (M->dim + 1)))) return rc;
     TimerScope ts(M->ctx, TK_UPDATE);
     rc = somhip_launch_update(M->ctx->stream, A->acc, M->topo, M->mn, M->dim, sigma, (double *)A->upd.p, nsplit, M->w,
                               history_out);

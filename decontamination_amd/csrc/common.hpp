@@ -207,6 +207,7 @@ struct somhip_ctx {
 struct somhip_model {
     somhip_ctx *ctx = nullptr;
     int m = 0, n = 0, mn = 0, dim = 0, dpad = 0, topology = SOMHIP_TOPOLOGY_HEXAGONAL;
+    bool separable = false; // topography is the closed-form square / hexagonal grid (separable neighbourhood)
     int nblk = 0;          // number of 32-neuron blocks in natural order (mn rounded up to 32)
     int pblk = 0;          // number of 32-slot blocks in the permuted (BF16) order
     int pblk_a = 0;        // leading permuted blocks holding the coarse sub-lattice (== pblk: none)

@@ -307,6 +307,74 @@ __global__ __launch_bounds__(256) void update_partial_kernel(const double *__res
     }
 }
 
+// Separable form of the same update on the reference's grids (neuron a*n + b at x = a (+ 0.5 (b % 2)
+// for hexagonal), y = b or b sqrt(3)/2, all FP32): H_jc = exp(-(fl(dx^2) + fl(dy^2)) inv2s2) =
+// gx(dx) gy(dy), and dx = (a_j - a_c) + 0.5 (p_j - p_c) with p the column parity (0 on square grids).
+// Pass 1, per source column b_c and destination parity p_j:
+//   T[b_c][p_j][a_j][k] = sum_{a_c} gx(a_j - a_c + 0.5 (p_j - p_c)) S[a_c n + b_c][k]
+// Pass 2, per destination column b_j: num[a_j n + b_j][k] = sum_{b_c} gy(y_j - y_c) T[b_c][p_j][a_j][k],
+// then w = num / den as divide_kernel does. m*n*(m+n)*(D+1) products instead of (m n)^2 (D+1).
+__global__ __launch_bounds__(256) void update_sep_rows_kernel(const double *__restrict__ S, int m, int n, int nv,
+                                                              int hex, double inv_2s2, double *__restrict__ T)
+{
+    extern __shared__ double sh[];
+    double *sS = sh;            // [m][nv] the source column
+    double *gx = sh + m * nv;   // [2m - 1] gx(a_j - a_c + delta), a_j - a_c = t - (m - 1)
+    const int np = hex ? 2 : 1;
+    const int bc = blockIdx.x / np, pj = blockIdx.x % np;
+    for (int e = threadIdx.x; e < m * nv; e += blockDim.x) {
+        const int a = e / nv, k = e % nv;
+        sS[e] = S[((int64_t)a * n + bc) * nv + k];
+    }
+    const float delta = hex ? 0.5f * (float)(pj - (bc & 1)) : 0.f;
+    for (int t = threadIdx.x; t < 2 * m - 1; t += blockDim.x) {
+        const float d = (float)(t - (m - 1)) + delta;  // exact: small halves
+        const float d2 = __fmul_rn(d, d);
+        gx[t] = exp(-(double)d2 * inv_2s2);
+    }
+    __syncthreads();
+    double *out = T + (int64_t)blockIdx.x * m * nv;
+    for (int e = threadIdx.x; e < m * nv; e += blockDim.x) {
+        const int aj = e / nv, k = e % nv;
+        double acc = 0.0;
+        for (int ac = 0; ac < m; ac++) acc = __fma_rn(gx[aj - ac + m - 1], sS[ac * nv + k], acc);
+        out[e] = acc;
+    }
+}
+
+__global__ __launch_bounds__(256) void update_sep_cols_kernel(const double *__restrict__ T, const float *__restrict__ topo,
+                                                              int m, int n, int dim, int hex, double inv_2s2,
+                                                              float *__restrict__ w, float *__restrict__ hist)
+{
+    extern __shared__ double sh[];
+    const int nv = dim + 1;
+    double *gy = sh;            // [n]
+    double *num = sh + n;       // [m][nv]
+    const int bj = blockIdx.x;
+    const int np = hex ? 2 : 1;
+    const int pj = hex ? (bj & 1) : 0;
+    const float yj = topo[2 * bj + 1];  // neuron (0, bj)
+    for (int t = threadIdx.x; t < n; t += blockDim.x) {
+        const float dy = __fsub_rn(yj, topo[2 * t + 1]);
+        gy[t] = exp(-(double)__fmul_rn(dy, dy) * inv_2s2);
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < m * nv; e += blockDim.x) {
+        double acc = 0.0;
+        for (int bc = 0; bc < n; bc++) acc = __fma_rn(gy[bc], T[((int64_t)(bc * np + pj) * m) * nv + e], acc);
+        num[e] = acc;
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < m * dim; e += blockDim.x) {
+        const int aj = e / dim, k = e % dim;
+        const double den = num[aj * nv + dim];
+        const float v = den > SOM_FLOAT_EPS ? (float)(num[aj * nv + k] / den) : 0.f;
+        const int64_t o = ((int64_t)aj * n + bj) * dim + k;
+        w[o] = v;
+        if (hist) hist[o] = v;
+    }
+}
+
 // w_j = num_j / den_j if den_j > 1e-12 else 0 (som_batch.py:354-359); num/den = fixed-order sums of
 // the split partials (or the accumulator itself when nsplit == 1 and part == acc).
 __global__ void divide_kernel(const double *__restrict__ part, int nsplit, int mn, int dim, float *__restrict__ w,
@@ -568,6 +636,32 @@ int somhip_launch_update(hipStream_t st, const double *S, const float *topo, int
     const int64_t tot = (int64_t)mn * dim;
     hipLaunchKernelGGL(divide_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, part, nsplit, mn, dim, w,
                        hist);
+    SOM_HIP_TRY(hipGetLastError());
+    return SOMHIP_OK;
+}
+
+bool somhip_update_separable_fits(int m, int n, int dim)
+{
+    const size_t nv = (size_t)dim + 1;
+    return sizeof(double) * ((size_t)m * nv + 2 * (size_t)m) <= 65536 &&
+           sizeof(double) * ((size_t)n + (size_t)m * nv) <= 65536;
+}
+
+int somhip_launch_update_separable(hipStream_t st, const double *S, const float *topo, int m, int n, int hex, int dim,
+                                   double sigma, double *T, float *w, float *hist)
+{
+    const int nv = dim + 1;
+    const double inv_2s2 = 1.0 / (2.0 * sigma * sigma);
+    const int np = hex ? 2 : 1;
+    const size_t lds1 = sizeof(double) * ((size_t)m * nv + 2 * m - 1);
+    const size_t lds2 = sizeof(double) * ((size_t)n + (size_t)m * nv);
+    if (!somhip_update_separable_fits(m, n, dim)) {
+        somhip_set_error("separable update: grid too large for one LDS column");
+        return SOMHIP_ERR_UNSUPPORTED;
+    }
+    hipLaunchKernelGGL(update_sep_rows_kernel, dim3(n * np), dim3(256), lds1, st, S, m, n, nv, hex, inv_2s2, T);
+    SOM_HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(update_sep_cols_kernel, dim3(n), dim3(256), lds2, st, T, topo, m, n, dim, hex, inv_2s2, w, hist);
     SOM_HIP_TRY(hipGetLastError());
     return SOMHIP_OK;
 }

@@ -20,6 +20,9 @@ int somhip_launch_scatter(hipStream_t st, const int32_t *bmu, const float *x, in
 int somhip_update_nsplit(int mn);
 int somhip_launch_update(hipStream_t st, const double *S, const float *topo, int mn, int dim, double sigma,
                          double *part, int nsplit, float *w, float *hist);
+bool somhip_update_separable_fits(int m, int n, int dim);
+int somhip_launch_update_separable(hipStream_t st, const double *S, const float *topo, int m, int n, int hex, int dim,
+                                   double sigma, double *T, float *w, float *hist);
 int somhip_launch_divide(hipStream_t st, const double *acc, int mn, int dim, float *w, float *hist);
 int somhip_iter_nsplit(int64_t n, int mn);
 int somhip_launch_iter(hipStream_t st, const int32_t *bmu, const float *x, int dim, int dpad, const double *rho,
