@@ -334,3 +334,31 @@ def test_cpu_configuration_in_full():
     assert np.abs(r.weights - W64).max() < np.abs(W32 - W64).max()
     assert np.allclose(r.quantization_errors, qe32, rtol=1e-3)
     assert np.array_equal(r.get_winners(X), R.find_bmus(r.weights, X, REF32))
+
+
+@pytest.mark.parametrize("grid", ["closed_form", "custom"])
+def test_update_paths_vs_fp64_oracle(grid):
+    """One epoch through the C ABI against the oracle's FP64 epoch step on the same topography. A model
+    with the reference's closed-form grid takes the separable update (rows then columns); any other
+    topography, here random FP32 coordinates, takes the dense H.S kernel."""
+    from decontamination_amd import _lib
+    from decontamination_amd.runtime import Accumulator, DeviceArray, Model, get_context
+    m, n, dim, N = 12, 15, 16, 30_000
+    rng = np.random.default_rng(91)
+    X = rng.standard_normal((N, dim)).astype(np.float32)
+    W0 = rng.standard_normal((m * n, dim)).astype(np.float32) * 0.5
+    topo = R.topography(m, n, "hexagonal")
+    if grid == "custom":
+        topo = (rng.random((m * n, 2)) * 10).astype(np.float32)
+    ctx = get_context(0)
+    model = Model(ctx, m, n, dim, "hexagonal", topo)
+    model.set_weights(W0)
+    acc = Accumulator(model)
+    xd = DeviceArray.from_host(ctx, X)
+    acc.reset()
+    acc.epoch_accumulate(xd.ptr, None, N, _lib.PREC_EXACT, _lib.RERANK_FP64)
+    acc.epoch_update(0, 3, 7.5)
+    got = model.get_weights()
+    num, den = R.epoch_accumulate(W0, topo, X, None, 0, 3, 7.5, FP64)
+    want = R.divide(num, den, FP64)
+    assert np.allclose(got, want, rtol=1e-6, atol=1e-7), np.abs(got - want).max()
