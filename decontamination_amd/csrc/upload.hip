@@ -25,12 +25,16 @@ constexpr size_t STAGE_PIECE = (size_t)32 << 20; // bytes per piece
 constexpr int COPY_THREADS = 8;                  // host threads per piece copy (including the caller)
 constexpr size_t STAGE_MIN = (size_t)4 << 20;    // smaller pageable copies go straight to the runtime
 
-// Process-wide helper threads for the memcpy of one piece, split in COPY_THREADS parts (the caller
-// takes part 0). Started on first use and kept until the process ends; callers take turns.
+// Process-wide staging: the pinned pieces and the helper threads that fill them (a memcpy of one
+// piece is split in COPY_THREADS parts; the caller takes part 0). Created on first use and kept until
+// the process ends, so any number of uploaders shares 128 MB of pinned memory; they take turns.
 struct CopyPool {
     std::vector<std::thread> th;
-    std::mutex use;  // one copy at a time
+    std::mutex use;  // held by one uploader's staged copy at a time
     std::mutex m;
+    char *stage[STAGE_PIECES] = {};
+    somhip_uploader *last[STAGE_PIECES] = {};  // uploader whose DMA read the piece last (null: none pending)
+    int next_piece = 0;
     std::condition_variable go, fin;
     char *dst = nullptr;
     const char *src = nullptr;
@@ -65,9 +69,9 @@ struct CopyPool {
                 }
             });
     }
+    // caller holds `use`
     void copy(char *d, const char *s, size_t bytes)
     {
-        std::lock_guard<std::mutex> turn(use);
         {
             std::lock_guard<std::mutex> l(m);
             dst = d; src = s; n = bytes;
@@ -95,6 +99,8 @@ CopyPool &copy_pool()
 
 struct somhip_uploader {
     somhip_ctx *ctx = nullptr;
+    hipEvent_t drained[STAGE_PIECES] = {};  // per staging piece: recorded on `copy` after this uploader's DMA
+    bool staged = false;                    // has used the staging pieces
     hipStream_t copy = nullptr;
     int slots = 0;
     int cur = -1;                   // slot handed out by the last put (-1: none yet)
@@ -102,11 +108,6 @@ struct somhip_uploader {
     std::vector<hipEvent_t> done;   // per slot: recorded on the compute stream after its last reader
     std::vector<hipEvent_t> ready;  // per slot: recorded on the copy stream after its copy
     std::vector<char> done_valid;
-    // pinned staging ring (allocated on the first pageable put)
-    char *stage[STAGE_PIECES] = {};
-    hipEvent_t drained[STAGE_PIECES] = {};  // recorded on the copy stream after the piece's DMA
-    bool drained_valid[STAGE_PIECES] = {};
-    int next_piece = 0;
 };
 
 namespace {
@@ -115,14 +116,17 @@ void uploader_release(somhip_uploader *U)
 {
     if (U->copy) (void)hipStreamSynchronize(U->copy);
     if (U->ctx) (void)hipStreamSynchronize(U->ctx->stream);
+    if (U->staged) {
+        CopyPool &P = copy_pool();
+        std::lock_guard<std::mutex> turn(P.use);
+        for (int p = 0; p < STAGE_PIECES; p++)
+            if (P.last[p] == U) P.last[p] = nullptr;  // its DMAs have finished with the stream
+    }
+    for (auto e : U->drained) if (e) (void)hipEventDestroy(e);
     for (auto &b : U->x) b.release();
     for (auto &b : U->rho) b.release();
     for (auto e : U->done) if (e) (void)hipEventDestroy(e);
     for (auto e : U->ready) if (e) (void)hipEventDestroy(e);
-    for (int p = 0; p < STAGE_PIECES; p++) {
-        if (U->stage[p]) (void)hipHostFree(U->stage[p]);
-        if (U->drained[p]) (void)hipEventDestroy(U->drained[p]);
-    }
     if (U->copy) (void)hipStreamDestroy(U->copy);
 }
 
@@ -146,21 +150,21 @@ int stage_copy(somhip_uploader *U, char *dst, const char *src, size_t bytes)
         SOM_HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, U->copy));
         return SOMHIP_OK;
     }
-    if (!U->stage[0]) {
-        for (int p = 0; p < STAGE_PIECES; p++) {
-            SOM_HIP_TRY(hipHostMalloc((void **)&U->stage[p], STAGE_PIECE, hipHostMallocDefault));
-            SOM_HIP_TRY(hipEventCreateWithFlags(&U->drained[p], hipEventDisableTiming));
-        }
-    }
+    CopyPool &P = copy_pool();
+    std::lock_guard<std::mutex> turn(P.use);
+    for (int p = 0; p < STAGE_PIECES; p++)
+        if (!P.stage[p]) SOM_HIP_TRY(hipHostMalloc((void **)&P.stage[p], STAGE_PIECE, hipHostMallocPortable));
     for (size_t off = 0; off < bytes; off += STAGE_PIECE) {
         const size_t len = std::min(STAGE_PIECE, bytes - off);
-        const int p = U->next_piece;
-        U->next_piece = (p + 1) % STAGE_PIECES;
-        if (U->drained_valid[p]) SOM_HIP_TRY(hipEventSynchronize(U->drained[p]));  // its last DMA is done
-        copy_pool().copy(U->stage[p], src + off, len);
-        SOM_HIP_TRY(hipMemcpyAsync(dst + off, U->stage[p], len, hipMemcpyHostToDevice, U->copy));
+        const int p = P.next_piece;
+        P.next_piece = (p + 1) % STAGE_PIECES;
+        if (P.last[p]) SOM_HIP_TRY(hipEventSynchronize(P.last[p]->drained[p]));  // its last DMA is done
+        P.last[p] = nullptr;
+        P.copy(P.stage[p], src + off, len);
+        SOM_HIP_TRY(hipMemcpyAsync(dst + off, P.stage[p], len, hipMemcpyHostToDevice, U->copy));
         SOM_HIP_TRY(hipEventRecord(U->drained[p], U->copy));
-        U->drained_valid[p] = true;
+        P.last[p] = U;
+        U->staged = true;
     }
     return SOMHIP_OK;
 }
@@ -190,6 +194,8 @@ int somhip_uploader_create(somhip_ctx *C, int slots, somhip_uploader **out)
         e = hipEventCreateWithFlags(&U->done[s], hipEventDisableTiming);
         if (e == hipSuccess) e = hipEventCreateWithFlags(&U->ready[s], hipEventDisableTiming);
     }
+    for (int p = 0; e == hipSuccess && p < STAGE_PIECES; p++)
+        e = hipEventCreateWithFlags(&U->drained[p], hipEventDisableTiming);
     if (e != hipSuccess) {
         somhip_set_error(std::string("somhip_uploader_create: ") + hipGetErrorString(e));
         uploader_release(U);

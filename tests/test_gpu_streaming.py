@@ -167,3 +167,29 @@ def test_large_chunk_wraps_the_staging_ring_and_pinned_chunks():
     act_pageable = som.get_activation_map(lambda: lambda: iter([X[:1_000_000], X[1_000_000:]]))
     want_act = np.bincount(want, minlength=m * n).reshape(m, n).astype(np.float32)
     assert np.array_equal(act_pinned, want_act) and np.array_equal(act_pageable, want_act)
+
+
+def test_two_uploaders_share_the_staging_ring():
+    """The pinned staging pieces are process-wide: uploaders that take turns (and one destroyed while
+    the other goes on) must each land their own bytes. 48 MB chunks take two pieces per put, so the
+    second uploader refills pieces whose last DMA belonged to the first."""
+    m, n, dim, N = 16, 16, 16, 750_000
+    print("two uploaders on the shared staging ring", flush=True)
+    rng = np.random.default_rng(72)
+    som = dam.SOM_Batch(m, n, dim)
+    som._weights[...] = rng.standard_normal((m * n, dim)).astype(np.float32)
+    model = som._device_model()
+    ctx = get_context(0)
+    chunks = [rng.standard_normal((N, dim), dtype=np.float32) for _ in range(6)]
+    want = [som.get_winners(c) for c in chunks]
+    a, b = Uploader(ctx, slots=2), Uploader(ctx, slots=2)
+    outs = [DeviceArray.empty(ctx, (N,), np.int32) for _ in chunks]
+    for k, c in enumerate(chunks):
+        if k == 4:
+            a.close()  # b goes on alone over pieces a used last
+        up = b if k % 2 or a.handle is None else a
+        xp, _ = up.put(c)
+        model.find_bmus(xp, N, outs[k].ptr, _lib.PREC_BF16_RERANK, _lib.RERANK_REF32)
+    b.close()
+    for k in range(len(chunks)):
+        assert np.array_equal(outs[k].copy_to_host(), want[k]), k
