@@ -488,6 +488,14 @@ __device__ __forceinline__ void bmu_bf16_body(const BmuModelDev &Md, const float
         for (int r = 0; r < 16; r += 2) t = fminf(fminf(t, v[r]), v[r + 1]);
         return t;
     };
+    // minima of the lane's four row quads (rows 4i..4i+3 are 4 consecutive neurons) and of the block;
+    // chains seeded with +inf, as in block_min, compile to v_min3 without canonicalising the inputs
+    auto quad_min = [](const f32x16 &v, float (&q)[4]) {
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+            q[i] = fminf(fminf(fminf(fminf(__builtin_inff(), v[4 * i]), v[4 * i + 1]), v[4 * i + 2]), v[4 * i + 3]);
+        return fminf(fminf(fminf(q[0], q[1]), q[2]), q[3]);
+    };
 
     // Own top-2 of this lane's half-block minima. The two lanes of a sample cover disjoint neuron
     // sets, so the merged top-2 are approximate distances of two distinct neurons.
@@ -520,35 +528,40 @@ __device__ __forceinline__ void bmu_bf16_body(const BmuModelDev &Md, const float
         lost[g] = false;
     }
     // Appends every row of this lane's half-block with D <= T; the pre-check skips the per-value
-    // compares when no lane of the wave has a hit in the block.
-    // The 16 row compares are issued back to back as wave masks; rows without any hit cost one scalar
-    // test. Past CAPL a lane's candidates go to its group's shared pool when there is one (slots handed
+    // compares when no lane of the wave has a hit in the block, and a second one per row quad skips
+    // the quads without a hit (hits in a block usually fall in one quad).
+    // Rows of a quad with a hit cost one wave compare and one scalar test each when none of their lanes
+    // hits. Past CAPL a lane's candidates go to its group's shared pool when there is one (slots handed
     // out in lane order from the wave's ballot), else the lane is lost to the fallback. Lanes without a
     // hit, and candidates past the list, write their own-list store to the spare last row.
-    auto record = [&](int blk, int g, const f32x16 &v, float t) {
+    auto record = [&](int blk, int g, const f32x16 &v, const float (&q)[4], float t) {
         if (__builtin_amdgcn_ballot_w64(t <= T[g]) != 0ull) {
             const int sbase_blk = blk * 32 + 4 * khalf;
-            unsigned long long bm[16];
+            unsigned long long bq[4];
 #pragma unroll
-            for (int r = 0; r < 16; r++) bm[r] = __builtin_amdgcn_ballot_w64(v[r] <= T[g]);
+            for (int i = 0; i < 4; i++) bq[i] = __builtin_amdgcn_ballot_w64(q[i] <= T[g]);
 #pragma unroll
-            for (int r = 0; r < 16; r++) {
-                if (__builtin_expect(bm[r] != 0ull, 0)) {  // uniform, rare: out of line
-                    const bool hit = v[r] <= T[g];  // lanes without a hit write to the spare row
-                    const unsigned short slot = (unsigned short)(sbase_blk + 8 * (r >> 2) + (r & 3));
-                    cw[g][hit ? min(cnt[g], CAPL) : CAPL][lane] = slot;
-                    if constexpr (POOL > 0) {
-                        const bool spill = hit && cnt[g] >= CAPL;
-                        const unsigned long long sp = __builtin_amdgcn_ballot_w64(spill);
-                        if (__builtin_expect(sp != 0ull, 0)) {  // uniform, rarer still
-                            const int off = pool_n[g] + (int)__builtin_amdgcn_mbcnt_hi(
-                                                (unsigned)(sp >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)sp, 0u));
-                            if (spill && off < POOL) sm.pool[wave][g][off] = ((unsigned)lane << 16) | slot;
-                            lost[g] = lost[g] || (spill && off >= POOL);
-                            pool_n[g] += __builtin_popcountll(sp);
+            for (int i = 0; i < 4; i++) {
+                if (bq[i] == 0ull) continue;  // uniform: no hit in this quad
+#pragma unroll
+                for (int r = 4 * i; r < 4 * i + 4; r++) {
+                    if (__builtin_expect(__builtin_amdgcn_ballot_w64(v[r] <= T[g]) != 0ull, 0)) {  // uniform, rare
+                        const bool hit = v[r] <= T[g];  // lanes without a hit write to the spare row
+                        const unsigned short slot = (unsigned short)(sbase_blk + 8 * (r >> 2) + (r & 3));
+                        cw[g][hit ? min(cnt[g], CAPL) : CAPL][lane] = slot;
+                        if constexpr (POOL > 0) {
+                            const bool spill = hit && cnt[g] >= CAPL;
+                            const unsigned long long sp = __builtin_amdgcn_ballot_w64(spill);
+                            if (__builtin_expect(sp != 0ull, 0)) {  // uniform, rarer still
+                                const int off = pool_n[g] + (int)__builtin_amdgcn_mbcnt_hi(
+                                                    (unsigned)(sp >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)sp, 0u));
+                                if (spill && off < POOL) sm.pool[wave][g][off] = ((unsigned)lane << 16) | slot;
+                                lost[g] = lost[g] || (spill && off >= POOL);
+                                pool_n[g] += __builtin_popcountll(sp);
+                            }
                         }
+                        cnt[g] += hit ? 1 : 0;
                     }
-                    cnt[g] += hit ? 1 : 0;
                 }
             }
         }
@@ -571,8 +584,9 @@ __device__ __forceinline__ void bmu_bf16_body(const BmuModelDev &Md, const float
         stream(nchunks_a, nchunks, 0, [&](int blk, f32x16 (&acc)[G]) {
 #pragma unroll
             for (int g = 0; g < G; g++) {
-                const float t = block_min(acc[g]);
-                record(blk, g, acc[g], t);
+                float q[4];
+                const float t = quad_min(acc[g], q);
+                record(blk, g, acc[g], q, t);
                 update(g, t);
             }
         }, refresh);
@@ -581,7 +595,11 @@ __device__ __forceinline__ void bmu_bf16_body(const BmuModelDev &Md, const float
     // ---------------- phase C: the coarse section again, recording under the final threshold ----------------
     stream(0, nchunks_a, -1, [&](int blk, f32x16 (&acc)[G]) {
 #pragma unroll
-        for (int g = 0; g < G; g++) record(blk, g, acc[g], block_min(acc[g]));
+        for (int g = 0; g < G; g++) {
+            float q[4];
+            const float t = quad_min(acc[g], q);
+            record(blk, g, acc[g], q, t);
+        }
     }, nothing);
 
     // ---------------- exact rerank of the candidate set ----------------
