@@ -101,7 +101,8 @@ int somhip_memset(somhip_ctx *ctx, void *dptr, int value, size_t bytes);
  * slot, after the kernels that read that slot last time round have finished, and makes the
  * context's stream wait for the copy. The copy of chunk k+1 thus overlaps the kernels of chunk k.
  * The returned device pointers stay valid until `slots` further puts. Pageable chunks of 4 MB or
- * more are staged through pinned pieces by host threads, overlapped with their DMA. Pageable host
+ * more are staged through pinned pieces by host threads, overlapped with their DMA; the 128 MB of
+ * pieces are allocated once per process and shared by all uploaders, which take turns. Pageable host
  * memory may be reused once put() returns; memory from somhip_host_alloc is copied directly and
  * must stay untouched until the context's stream has passed the kernels that read the chunk. */
 typedef struct somhip_uploader somhip_uploader;
