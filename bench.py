@@ -517,8 +517,9 @@ def serving_bench(X16, seconds, rate, budget_gb, seed=0):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU; run with --gpus 1 (multi-GPU sharding is "
-                                   "described in DESIGN.md, not run)"}), flush=True)
+        print(json.dumps({"error": "bench.py measures one GPU; run with --gpus 1 (training would shard over "
+                                   "samples with one exchange of the per-neuron sums per epoch; not "
+                                   "implemented)"}), flush=True)
         sys.exit(2)
     import decontamination_amd as dam
     from decontamination_amd import synth
