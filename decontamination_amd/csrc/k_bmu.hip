@@ -11,10 +11,10 @@
 //   FP32        bmu_fp32_kernel: |w'|^2 - 2 x'.w' on v_mfma_f32_32x32x2_f32 (x', w' centred on the
 //               weight centroid), top-2 on those values (the plain FP32 form).
 //   BF16_RERANK bmu_bf16_kernel: the same GEMM on v_mfma_f32_32x32x16_bf16 with a 3-term split
-//               (x_h w_h + x_l w_h + x_h w_l). Phase A scans a coarse sub-lattice of the map for the
-//               top-2 of per-lane block minima; phase B scans the other neurons, recording every
-//               neuron inside a provable error window of the running second minimum while it
-//               tightens the window; phase C records the coarse neurons under the final window.
+//               (x_h w_h + x_l w_h + x_h w_l). Phase A scans half of a coarse sub-lattice of the
+//               map for the top-2 of per-lane block minima; phase B scans every other neuron,
+//               recording every neuron inside a provable error window of the running second minimum
+//               while it tightens the window; phase C records phase A's neurons under the final window.
 //               Then the exact distance formula on the recorded set. Samples whose list overflows,
 //               or whose bound is not finite, go to the EXACT fallback.
 #include "common.hpp"
@@ -569,19 +569,20 @@ __device__ __forceinline__ void bmu_bf16_body(const BmuModelDev &Md, const float
     auto nothing = [] {};
     const int nchunks_a = (Md.nblk_a + CB - 1) / CB;
 
-    // ---------------- phase A: coarse section, top-2 only ----------------
-    stream(0, nchunks_a, nchunks_a < nchunks ? nchunks_a : 0, [&](int, f32x16 (&acc)[G]) {
+    // ---------------- phase A: the first half of the coarse section, top-2 only ----------------
+    const int nchunks_a1 = max(1, nchunks_a / 2);
+    stream(0, nchunks_a1, nchunks_a1 < nchunks ? nchunks_a1 : 0, [&](int, f32x16 (&acc)[G]) {
 #pragma unroll
         for (int g = 0; g < G; g++) update(g, block_min(acc[g]));
     }, nothing);
     refresh();
 
-    // ---------------- phase B: the other blocks, recording under a running threshold ----------------
-    // A block is recorded against the threshold in force before its own minima are merged; the
-    // threshold in force at any time is >= the final one, so the recorded set contains every
-    // neuron with D <= final T.
-    if (nchunks_a < nchunks) {
-        stream(nchunks_a, nchunks, 0, [&](int blk, f32x16 (&acc)[G]) {
+    // ---------------- phase B: every other block, recording under a running threshold ----------------
+    // The rest of the coarse section comes first, then the other blocks. A block is recorded against
+    // the threshold in force before its own minima are merged; the threshold in force at any time is
+    // >= the final one, so the recorded set contains every neuron with D <= final T.
+    if (nchunks_a1 < nchunks) {
+        stream(nchunks_a1, nchunks, 0, [&](int blk, f32x16 (&acc)[G]) {
 #pragma unroll
             for (int g = 0; g < G; g++) {
                 float q[4];
@@ -592,8 +593,8 @@ __device__ __forceinline__ void bmu_bf16_body(const BmuModelDev &Md, const float
         }, refresh);
     }
 
-    // ---------------- phase C: the coarse section again, recording under the final threshold ----------------
-    stream(0, nchunks_a, -1, [&](int blk, f32x16 (&acc)[G]) {
+    // ---------------- phase C: phase A's chunks again, recording under the final threshold ----------------
+    stream(0, nchunks_a1, -1, [&](int blk, f32x16 (&acc)[G]) {
 #pragma unroll
         for (int g = 0; g < G; g++) {
             float q[4];
