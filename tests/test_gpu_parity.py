@@ -93,6 +93,10 @@ CASES = [
     (45, 50, 32, 3000, "square", 12),
     (20, 30, 32, 3000, "square", 9),
     (7, 9, 40, 700, "square", 10),
+    # coarse sections of several chunks: 3 at 80x80/D16; 10 four-block chunks at 100x100/D32, so
+    # the BF16 search rescans 5 and records the other 5 in its main pass
+    (80, 80, 16, 4000, "hexagonal", 13),
+    (100, 100, 32, 3000, "hexagonal", 14),
 ]
 
 
