@@ -280,5 +280,41 @@ class Decontamination_SOM(object):
                                     _materialise(self._corrected_galaxy_number_density), self._som.m, self._som.n,
                                     self._som.dtype)
 
+    # -- HDF5 model format (decontamination_som.py:451-537) --------------------------------------
+    def save(self, filename: str) -> None:
+        """Saves the trained decontamination model: the PCA to ``<name>_pca.hdf5``, the SOM's ``model``
+        group to ``filename``, then a ``dataset`` group with nside, footprint, coverage,
+        footprint_systematics, galaxy_number_density and the footprint winners (``winner``). Datasets
+        given as generator builders are written as the arrays their chunks concatenate to."""
+        import h5py
+        self._pca.save(filename.replace('.hdf5', '_pca.hdf5'))
+        self._som.save(filename)
+        fields = (('nside', self._nside), ('footprint', self._footprint), ('coverage', self._coverage),
+                  ('footprint_systematics', _materialise(self._footprint_systematics)),
+                  ('galaxy_number_density', _materialise(self._galaxy_number_density)),
+                  ('winner', self._winners))
+        with h5py.File(filename, 'r+') as file:
+            group = file.create_group('dataset')
+            for name, data in fields:
+                group.create_dataset(name, data=data)
+        self._corrected_galaxy_number_density = ds_divide(_materialise(self._galaxy_number_density),
+                                                          np.asarray(self._coverage))
+
+    def load(self, filename: str) -> None:
+        """Loads a model written by ``save``: the PCA, the SOM and the ``dataset`` group. nside is read
+        as the scalar it was written as (the arrays with ``[:]``)."""
+        import h5py
+        self._pca.load(filename.replace('.hdf5', '_pca.hdf5'))
+        self._som.load(filename)
+        with h5py.File(filename, 'r') as file:
+            group = file['dataset']
+            self._nside = group['nside'][()]
+            self._footprint = group['footprint'][:]
+            self._coverage = group['coverage'][:]
+            self._footprint_systematics = group['footprint_systematics'][:]
+            self._galaxy_number_density = group['galaxy_number_density'][:]
+            self._winners = group['winner'][:]
+        self._corrected_galaxy_number_density = ds_divide(self._galaxy_number_density, self._coverage)
+
     def compute_clustered_gndm(self, *args, **kwargs):
         raise NotImplementedError('neuron clustering is not part of decontamination_amd')

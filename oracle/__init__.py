@@ -58,6 +58,8 @@ def lib():
         "orc_train_epoch_accumulate_omp": [P, P, P, P, P, P, I64, I64, I64, F, I, I],
         "orc_compute_errors_omp": [P, P, I, I, P, P, I64, D, P],
         "orc_welford_update": [P, P, P, P, P, P, I64, I],
+        "orc_top2_omp": [I, P, I, I, P, I64, P, P, P],
+        "orc_count_bmus_omp": [I, P, I, I, P, P, I64, P, P, I],
     }
     for name, argtypes in sigs.items():
         fn = getattr(L, name)

@@ -30,6 +30,7 @@
  * timed CPU baseline. Their summation order depends on thread timing, like the reference's.
  */
 #include <math.h>
+#include <omp.h>
 #include <stdint.h>
 #include <string.h>
 
@@ -264,6 +265,55 @@ void orc_compute_errors_omp(const float *w, const float *topo, int mn, int dim, 
     errors[0] += e0;
     errors[1] += e1;
     errors[2] += e2;
+}
+
+/* Per-sample top-2 of the _compute_errors_xpu rule (som_abstract.py:832-853) with the distance of the
+ * first, parallel over samples (each sample is independent, so the result equals the sequential
+ * loop's). c1 = -1 (d1 = NaN) when no distance is finite, c2 = -1 when there is no second. Used by the
+ * tests and the accuracy legs of bench.py to check full-size outputs. */
+void orc_top2_omp(int mode, const float *w, int mn, int dim, const float *x, int64_t n, int32_t *c1, int32_t *c2,
+                  double *d1)
+{
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int64_t i = 0; i < n; i++) {
+        const float *xi = x + (size_t)i * dim;
+        double md2 = 1.0e99, md1 = 1.0e99;
+        int mi2 = -1, mi1 = -1;
+        for (int j = 0; j < mn; j++) {
+            double d = sqd(mode, w + (size_t)j * dim, xi, dim);
+            if (d != d) continue;
+            if (md1 > d) {
+                if (j != mi1) { md2 = md1; mi2 = mi1; }
+                md1 = d;
+                mi1 = j;
+            } else if (j != mi1 && md2 > d) {
+                md2 = d;
+                mi2 = j;
+            }
+        }
+        c1[i] = mi1;
+        c2[i] = mi2;
+        d1[i] = mi1 >= 0 ? md1 : NAN;
+    }
+}
+
+/* Parallel restatement of _count_bmus_kernel (som_abstract.py:699-732, nb.prange over samples with FP64
+ * atomic adds) for the CPU baseline: per-thread FP64 maps summed at the end. */
+void orc_count_bmus_omp(int mode, const float *w, int mn, int dim, const float *x, const double *rho, int64_t n,
+                        double *result, double *thread_maps, int max_threads)
+{
+#pragma omp parallel num_threads(max_threads)
+    {
+        double *loc = thread_maps + (size_t)omp_get_thread_num() * mn;
+        for (int j = 0; j < mn; j++) loc[j] = 0.0;
+#pragma omp for schedule(static)
+        for (int64_t i = 0; i < n; i++) {
+            double r = rho ? rho[i] : 1.0;
+            if (r > 0.0) loc[find_bmu(mode, w, x + (size_t)i * dim, mn, dim, 0)] += r;
+        }
+    }
+    for (int t = 0; t < max_threads; t++)
+        for (int j = 0; j < mn; j++) result[j] += thread_maps[(size_t)t * mn + j];
 }
 
 /* Covariance.compute's sample loop (decontamination/algo/covariance.py:30-84, _update_welford_cov_sums):

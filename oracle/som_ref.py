@@ -193,11 +193,43 @@ def top2_fp64_np(W, X, block=8192, k=4):
     return c1, c2, d1
 
 
-def neighbourhood_fp64(topo, sigma):
-    """H[j, c] = exp(-|t_j - t_c|^2 / (2 sigma^2)) with the reference's FP32 topography differences."""
+def top2_omp(W, X, mode=FP64):
+    """Per-sample top-2 (c1, c2, d1) of the _compute_errors_xpu rule (som_abstract.py:832-853), exact
+    distance formula in `mode` for every neuron, parallel over samples (orc_top2_omp)."""
+    W = f32(W); X = f32(X)
+    n = X.shape[0]
+    c1 = np.empty(n, np.int32); c2 = np.empty(n, np.int32); d1 = np.empty(n, np.float64)
+    lib().orc_top2_omp(mode, ptr(W), W.shape[0], W.shape[1], ptr(X), n, ptr(c1), ptr(c2), ptr(d1))
+    return c1, c2, d1
+
+
+def trajectory_fp64(W0, topo, X, sigma0, n_epochs):
+    """The epoch-mode schedule (som_batch.py:214-375 with _train_step2_xpu in its factorised form) in FP64
+    arithmetic from W0: exact FP64 BMUs of every sample, FP64 per-BMU sums, num = H.S, den = H.c, division;
+    the weights are rounded to FP32 after every epoch (the reference keeps them in its dtype, FP32).
+    Returns (final FP32 weights, BMUs of the last epoch, QE of the final weights)."""
+    W = f32(W0)
+    c1 = None
+    sq = topo_sqdist(topo)
+    for e in range(n_epochs):
+        c1, _, _ = top2_omp(W, X, FP64)
+        W, _, _, _ = factorised_epoch_fp64(W, topo, X, None, sigma_at(sigma0, e, n_epochs), bmu=c1,
+                                           out_dtype=np.float32, sq=sq)
+    _, _, d1 = top2_omp(W, X, FP64)
+    return W, c1, float(np.mean(np.sqrt(d1)))
+
+
+def topo_sqdist(topo):
+    """|t_j - t_c|^2 with the reference's FP32 differences and squares, summed in FP64 (a5 on the grid)."""
     t = np.asarray(topo, np.float32)
     diff = (t[:, None, :] - t[None, :, :]).astype(np.float32)
-    sq = (diff * diff).astype(np.float32).astype(np.float64).sum(-1)
+    return (diff * diff).astype(np.float32).astype(np.float64).sum(-1)
+
+
+def neighbourhood_fp64(topo, sigma, sq=None):
+    """H[j, c] = exp(-|t_j - t_c|^2 / (2 sigma^2)) with the reference's FP32 topography differences."""
+    if sq is None:
+        sq = topo_sqdist(topo)
     return np.exp(-sq * (1.0 / (2.0 * sigma * sigma)))
 
 
@@ -206,7 +238,7 @@ def sigma_at(sigma0, e, etot):
     return max(s, FLOAT_EPS)
 
 
-def factorised_epoch_fp64(W, topo, X, rho, sigma, bmu=None, out_dtype=np.float32):
+def factorised_epoch_fp64(W, topo, X, rho, sigma, bmu=None, out_dtype=np.float32, sq=None):
     """One FP64 batch epoch in the factorised form num = H.S, den = H.c (algebraically identical to
     som_batch.py:509-566 in epoch mode because h depends only on (j, bmu(i))). Returns
     (W_new in out_dtype, bmu, c2, d1)."""
@@ -217,7 +249,7 @@ def factorised_epoch_fp64(W, topo, X, rho, sigma, bmu=None, out_dtype=np.float32
     xk = np.asarray(X, np.float32)[keep].astype(np.float64)
     S = np.stack([np.bincount(c1[keep], weights=xk[:, k] * r[keep], minlength=mn) for k in range(dim)], 1)
     cnt = np.bincount(c1[keep], weights=r[keep], minlength=mn)
-    H = neighbourhood_fp64(topo, sigma)
+    H = neighbourhood_fp64(topo, sigma, sq)
     num = H @ S
     den = H @ cnt
     Wn = np.where(den[:, None] > FLOAT_EPS, num / np.where(den[:, None] > FLOAT_EPS, den[:, None], 1.0), 0.0)
@@ -292,3 +324,15 @@ def gndm(catalog_activation_map, footprint_activation_map, winners, m, n, dtype=
                     out=np.full(catalog_activation_map.shape, np.nan, dtype=dtype),
                     where=footprint_activation_map > 0.0)
     return n_gal, n_pix, gnd, gnd.reshape(m * n)[winners]
+
+
+def activation_map_omp(W, X, rho=None, mode=REF32, threads=None):
+    """activation_map, parallel over samples (orc_count_bmus_omp): per-thread FP64 maps summed at the end.
+    For unit weights the sums are integer counts, equal to the sequential loop's."""
+    import os
+    W = f32(W); X = f32(X); rho = f64(rho)
+    t = int(threads or os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    res = np.zeros(W.shape[0], np.float64)
+    tm = np.zeros((t, W.shape[0]), np.float64)
+    lib().orc_count_bmus_omp(mode, ptr(W), W.shape[0], W.shape[1], ptr(X), ptr(rho), X.shape[0], ptr(res), ptr(tm), t)
+    return res

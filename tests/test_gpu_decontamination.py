@@ -18,6 +18,7 @@ import decontamination_amd as dam
 from decontamination_amd import _lib
 from decontamination_amd.decontamination_som import compute_gndm
 from decontamination_amd.runtime import DeviceArray, get_context
+from h5stub import fake_h5py  # noqa: F401  (pytest fixture)
 from oracle import REF32, build as build_oracle
 from oracle import som_ref as R
 
@@ -173,3 +174,34 @@ def test_decontamination_pipeline_vs_oracle():
     assert np.array_equal(d.gnd, gnd, equal_nan=True) and np.array_equal(d.gndm, gndm, equal_nan=True)
     assert d.n_gal == n_gal and d.n_pix == n_pix
     assert np.allclose(d.visibility, (1.0 + (gndm - n_gal / n_pix) / (n_gal / n_pix)) * coverage, equal_nan=True)
+
+
+def test_decontamination_save_load_round_trip(fake_h5py):
+    """Decontamination_SOM.save/load (decontamination_som.py:451-537) through the in-memory h5py stand-in:
+    a model trained on the GPU, saved, loaded into a fresh object, re-infers bit-identical winners and
+    galaxy-number-density products."""
+    m, n, dim, N = 8, 8, 6, 30_000
+    print("decontamination save/load round trip", flush=True)
+    rng = np.random.default_rng(21)
+    X = (rng.standard_normal((N, dim)) * [1, 2, 0.5, 1, 3, 1.5]).astype(np.float32)
+    density = rng.poisson(3.0, N).astype(np.float64)
+    coverage = rng.random(N) * 0.5 + 0.5
+    d = dam.Decontamination_SOM(512, np.arange(N), coverage, X, density, m, n, dim, True)
+    d.train(n_epochs=3, show_progress_bar=False)
+    d.compute_gndm(show_progress_bar=False)
+    d.save("dec.hdf5")
+    assert set(fake_h5py) == {"dec.hdf5", "dec_pca.hdf5"}
+    assert set(fake_h5py["dec.hdf5"]["dataset"].data) == {"nside", "footprint", "coverage", "footprint_systematics",
+                                                          "galaxy_number_density", "winner"}
+    e = dam.Decontamination_SOM(1, np.zeros(1), np.ones(1), np.zeros((1, dim), np.float32), np.zeros(1), m, n, dim,
+                                True)
+    e.load("dec.hdf5")
+    assert e.nside == 512 and np.array_equal(e.footprint, np.arange(N)) and np.array_equal(e.coverage, coverage)
+    assert np.array_equal(e.weights, d.weights) and np.array_equal(e.pca.weights, d.pca.weights)
+    assert np.array_equal(e.winners, d.winners)
+    assert np.array_equal(e.som.get_winners(e.footprint_systematics), d.winners)
+    e.compute_gndm(show_progress_bar=False)
+    assert np.array_equal(e.catalog_activation_map, d.catalog_activation_map)
+    assert np.array_equal(e.footprint_activation_map, d.footprint_activation_map)
+    assert np.array_equal(e.gnd, d.gnd, equal_nan=True) and np.array_equal(e.gndm, d.gndm, equal_nan=True)
+    assert e.n_gal == d.n_gal and e.n_pix == d.n_pix

@@ -1,72 +1,18 @@
 """CPU: the HDF5 model format's host logic (SOM_Abstract.save/load, som_abstract.py:163-319).
 
-h5py is not installed in this image, so these tests put an in-memory stand-in module under the name
-``h5py`` for the duration of each test. It keeps what ``File``/``create_group``/``attrs``/
-``create_dataset`` receive and returns numbers as NumPy scalars, as h5py reads attributes back. What
+h5py is not installed in this image, so these tests put an in-memory stand-in module (tests/h5stub.py)
+under the name ``h5py`` for the duration of each test. It keeps what ``File``/``create_group``/
+``attrs``/``create_dataset`` receive and returns numbers as NumPy scalars, as h5py reads attributes back. What
 they check is this package's own logic: which header attributes and datasets are written, the
 int/float conversion of attributes on load, the cast of datasets to the model's dtype, missing keys
 left untouched, and the topography rebuilt from the loaded topology. The on-disk HDF5 layout itself
 is not pinned here (no h5py to write or read a real file).
 """
-import sys
-import types
-
 import numpy as np
 import pytest
 
 import decontamination_amd as dam
-
-
-class _Group:
-    def __init__(self):
-        self.attrs = _Attrs()
-        self.data = {}
-
-    def create_dataset(self, name, data, shape, dtype):
-        self.data[name] = np.array(data, dtype=dtype).reshape(shape)
-
-    def __getitem__(self, name):
-        return self.data[name]
-
-
-class _Attrs(dict):
-    def __setitem__(self, key, value):
-        # h5py stores Python scalars as NumPy scalars and returns them as such; strings stay str
-        super().__setitem__(key, value if isinstance(value, str) else np.asarray(value)[()])
-
-
-class _File:
-    files = {}
-
-    def __init__(self, filename, mode):
-        self.filename = filename
-        if mode == 'w':
-            _File.files[filename] = {}
-        elif filename not in _File.files:
-            raise FileNotFoundError(filename)
-
-    def create_group(self, name, track_order=False):
-        g = _Group()
-        _File.files[self.filename][name] = g
-        return g
-
-    def __getitem__(self, name):
-        return _File.files[self.filename][name]
-
-    def __enter__(self):
-        return self
-
-    def __exit__(self, *exc):
-        return False
-
-
-@pytest.fixture
-def fake_h5py(monkeypatch):
-    mod = types.ModuleType('h5py')
-    mod.File = _File
-    _File.files = {}
-    monkeypatch.setitem(sys.modules, 'h5py', mod)
-    return _File.files
+from h5stub import fake_h5py  # noqa: F401  (pytest fixture)
 
 
 def test_som_batch_round_trip(fake_h5py):
@@ -127,3 +73,36 @@ def test_missing_keys_are_left_untouched(fake_h5py):
 def test_load_of_a_missing_file_raises(fake_h5py):
     with pytest.raises(FileNotFoundError):
         dam.SOM_Batch(3, 3, 2).load('absent.h5')
+
+
+def test_decontamination_som_files_and_dataset_group(fake_h5py):
+    """Decontamination_SOM.save writes the PCA beside the SOM (`_pca.hdf5`) and a `dataset` group with the
+    reference's names (decontamination_som.py:451-489); generator builders are written as their arrays;
+    load reads nside back as a scalar and re-derives the corrected density."""
+    N, dim = 50, 3
+    rng = np.random.default_rng(4)
+    X = rng.standard_normal((N, dim)).astype(np.float32)
+    gal = rng.random(N) * 4
+    cov = rng.random(N) * 0.5 + 0.5
+    chunks = lambda: lambda: (X[a:a + 20] for a in range(0, N, 20))  # noqa: E731
+    d = dam.Decontamination_SOM(64, np.arange(N), cov, chunks, gal, 3, 4, dim, True)
+    d.som.init_rand(seed=5)
+    d.pca.init_rand(seed=6)
+    d._winners = rng.integers(0, 12, N).astype(np.int32)
+    d.save('run.hdf5')
+    assert set(fake_h5py) == {'run.hdf5', 'run_pca.hdf5'}
+    assert set(fake_h5py['run.hdf5']) == {'model', 'dataset'}
+    g = fake_h5py['run.hdf5']['dataset']
+    assert set(g.data) == {'nside', 'footprint', 'coverage', 'footprint_systematics', 'galaxy_number_density',
+                           'winner'}
+    assert np.array_equal(g.data['footprint_systematics'], X)
+    assert fake_h5py['run_pca.hdf5']['model'].attrs['mode'] == 'pca'
+    e = dam.Decontamination_SOM(1, np.zeros(1), np.ones(1), np.zeros((1, dim), np.float32), np.zeros(1), 3, 4, dim,
+                                True)
+    e.load('run.hdf5')
+    assert e.nside == 64 and np.ndim(e.nside) == 0
+    assert np.array_equal(e.footprint_systematics, X) and np.array_equal(e.winners, d.winners)
+    assert np.array_equal(e.weights, d.weights) and np.array_equal(e.pca.weights, d.pca.weights)
+    assert np.array_equal(e.corrected_galaxy_number_density, gal / cov)
+    with pytest.raises(ValueError):
+        fake_h5py['run.hdf5']['dataset']['nside'][:]  # a scalar dataset does not slice, as in h5py
