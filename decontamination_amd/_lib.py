@@ -30,6 +30,8 @@ PREC_BF16_RERANK = 2
 RERANK_REF32 = 0
 RERANK_FP64 = 1
 
+NOALLOC_MAX_ROWS = 1 << 21  # somhip_find_bmus_noalloc's row limit (SOMHIP_NOALLOC_MAX_ROWS)
+
 SERVE_WINNERS = 0
 SERVE_ACTIVATION = 1
 
@@ -71,6 +73,7 @@ SIGNATURES = {
     "somhip_model_get_weights": [P, P],
     "somhip_model_weights_ptr": [P, PP],
     "somhip_find_bmus": [P, P, I64, I, I, P],
+    "somhip_find_bmus_noalloc": [P, P, I64, I, I, P],
     "somhip_count_bmus": [P, P, P, I64, I, I, P],
     "somhip_compute_errors": [P, P, P, I64, I, I, P],
     "somhip_gndm": [P, P, P, I64, I, P, P, P, P],

@@ -146,6 +146,21 @@ int somhip_timer_collect(somhip_ctx *C)
     return SOMHIP_OK;
 }
 
+// Scratch of somhip_find_bmus_noalloc: a fallback counter (first 256 bytes) and a list of up to
+// SOMHIP_NOALLOC_MAX_ROWS int64 entries. Every device holds its own copy of this array for the life of the
+// process; host code reaches it only through the address hipGetSymbolAddress returns on that device
+// (kept in the context), never through the array's name.
+static __device__ __align__(256) unsigned char scratch[256 + sizeof(int64_t) * (size_t)SOMHIP_NOALLOC_MAX_ROWS];
+
+// Suspends the context's kernel timers for the duration of a call (event creation and the pending list
+// would allocate).
+struct TimingOff {
+    somhip_ctx *C;
+    bool was;
+    explicit TimingOff(somhip_ctx *c) : C(c), was(c->timer.enabled) { C->timer.enabled = false; }
+    ~TimingOff() { C->timer.enabled = was; }
+};
+
 struct somhip_accum {
     somhip_model *model = nullptr;
     int mode = 0;  // 0 empty, 1 epoch sums, 2 vector-count numerator/denominator
@@ -203,6 +218,14 @@ int somhip_ctx_create(int device, somhip_ctx **out)
         return SOMHIP_ERR_HIP;
     }
     C->own_stream = C->stream;
+    // this device's copy of the allocation-free entry point's scratch array (the device is current)
+    e = hipGetSymbolAddress(&C->noalloc_scratch, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) {
+        (void)hipStreamDestroy(C->own_stream);
+        delete C;
+        somhip_set_error(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        return SOMHIP_ERR_HIP;
+    }
     *out = C;
     return SOMHIP_OK;
 }
@@ -453,6 +476,31 @@ int somhip_find_bmus(somhip_model *M, const float *x, int64_t n, int precision, 
         return SOMHIP_ERR_ARG;
     }
     return somhip_launch_bmu(M, x, n, precision, arithmetic, 0, 0, out, nullptr, nullptr, nullptr);
+}
+
+int somhip_find_bmus_noalloc(somhip_model *M, const float *x, int64_t n, int precision, int arithmetic, int32_t *out)
+{
+    int rc = check_model(M);
+    if (rc) return rc;
+    if ((rc = check_prec(precision, arithmetic))) return rc;
+    if (n < 0 || (n > 0 && (!x || !out))) {
+        somhip_set_error("invalid x/out/n");
+        return SOMHIP_ERR_ARG;
+    }
+    if (n > SOMHIP_NOALLOC_MAX_ROWS) {
+        somhip_set_error("somhip_find_bmus_noalloc takes at most " + std::to_string(SOMHIP_NOALLOC_MAX_ROWS) +
+                         " rows per call (got " + std::to_string(n) + "); split the input");
+        return SOMHIP_ERR_ARG;
+    }
+    somhip_ctx *C = M->ctx;
+    if (!C->noalloc_scratch) {
+        somhip_set_error("context has no scratch address");
+        return SOMHIP_ERR_STATE;
+    }
+    TimingOff quiet(C);
+    unsigned int *fb_count = (unsigned int *)C->noalloc_scratch;
+    int64_t *fb_list = (int64_t *)((unsigned char *)C->noalloc_scratch + 256);
+    return somhip_launch_bmu(M, x, n, precision, arithmetic, 0, 0, out, nullptr, nullptr, nullptr, fb_list, fb_count);
 }
 
 int somhip_count_bmus(somhip_model *M, const float *x, const double *rho, int64_t n, int precision, int arithmetic,

@@ -201,6 +201,7 @@ struct somhip_ctx {
     int num_cus = 256;
     size_t lds_per_block = 65536;  // device limit of LDS bytes per workgroup
     Workspace ws;  // inference entry points
+    void *noalloc_scratch = nullptr;  // this device's copy of the allocation-free entry point's scratch array
 };
 
 // Derived per-weights data prepared by the prep kernels before any BMU search.
@@ -240,8 +241,11 @@ int somhip_timer_collect(somhip_ctx *C);
 int somhip_prepare(somhip_model *mdl);
 // xmax_part (nullable): per-wave rows of dpad floats with max |x_k| over finite values, for the
 // fixed-point scales of training; somhip_bmu_xmax_rows gives the number of rows written.
+// fb_list / fb_count (nullable, both or neither): the BF16 path's fallback workspace (n int64 entries and
+// one counter); when absent the context's grow-only workspace is used (and allocated if too small).
 int somhip_launch_bmu(somhip_model *mdl, const float *x, int64_t n, int precision, int arithmetic, int need_c2,
-                      int default_index, int32_t *c1, int32_t *c2, double *d1, float *xmax_part);
+                      int default_index, int32_t *c1, int32_t *c2, double *d1, float *xmax_part,
+                      int64_t *fb_list = nullptr, unsigned int *fb_count = nullptr);
 int64_t somhip_bmu_xmax_rows(int precision, int dpad, int64_t n);
 BmuModelDev somhip_model_dev(const somhip_model *M);
 // Serving form of the BF16 search: `grid` tiles from `tiles` (n == 0 tiles are skipped), then the exact

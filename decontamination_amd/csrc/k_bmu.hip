@@ -1121,7 +1121,8 @@ int64_t somhip_bmu_xmax_rows(int precision, int dpad, int64_t n)
 }
 
 int somhip_launch_bmu(somhip_model *M, const float *x, int64_t n, int precision, int arithmetic, int need_c2,
-                      int default_index, int32_t *c1, int32_t *c2, double *d1, float *xmax_part)
+                      int default_index, int32_t *c1, int32_t *c2, double *d1, float *xmax_part, int64_t *fb_list_ext,
+                      unsigned int *fb_count_ext)
 {
     if (n <= 0) return SOMHIP_OK;
     hipStream_t st = M->ctx->stream;
@@ -1159,11 +1160,16 @@ int somhip_launch_bmu(somhip_model *M, const float *x, int64_t n, int precision,
         somhip_set_error("BF16_RERANK supports at most 65536 neurons");
         return SOMHIP_ERR_UNSUPPORTED;
     }
-    int64_t *fb_list;
-    unsigned int *fb_count;
-    rc = fb_workspace(M->ctx, n, &fb_list, &fb_count);
-    if (rc) return rc;
-    SOM_HIP_TRY(hipMemsetAsync(fb_count, 0, sizeof(unsigned int), st));
+    int64_t *fb_list = fb_list_ext;
+    unsigned int *fb_count = fb_count_ext;
+    if (fb_list) {
+        // caller's workspace: cleared by a kernel, so the call holds no memset node under graph capture
+        if ((rc = somhip_launch_zero_u32(st, fb_count))) return rc;
+    } else {
+        rc = fb_workspace(M->ctx, n, &fb_list, &fb_count);
+        if (rc) return rc;
+        SOM_HIP_TRY(hipMemsetAsync(fb_count, 0, sizeof(unsigned int), st));
+    }
     {
         TimerScope ts(M->ctx, TK_BMU_BF16);
         if (dp == 16) launch_bf16<16>(arithmetic, need_c2, st, M, x, n, default_index, c1, c2, d1, fb_list, fb_count, xmax_part);

@@ -130,6 +130,19 @@ int somhip_model_weights_ptr(somhip_model *model, float **dptr);
  * x: device (n, dim) FP32; out: device int32 (n). */
 int somhip_find_bmus(somhip_model *model, const float *x, int64_t n, int precision, int arithmetic, int32_t *out);
 
+/* Winners, as somhip_find_bmus, without allocating: no hipMalloc/hipFree or other allocation inside the
+ * call (kernel timers are suspended for its duration). No counterpart in the reference: it is for callers
+ * that cannot allocate, such as code under HIP graph capture and serving loops. The BF16 path's fallback
+ * list lives in a fixed device array of 256 + 8 * SOMHIP_NOALLOC_MAX_ROWS bytes (16 MB) that every device
+ * holds for the life of the process, cleared by a kernel at the start of each call; n above
+ * SOMHIP_NOALLOC_MAX_ROWS is rejected with SOMHIP_ERR_ARG (split the input). All calls on one device share
+ * that array: calls that overlap on one device (several contexts or streams at once) race. When the weights
+ * changed since the last search, the call first rebuilds the model's derived data with the prep kernels,
+ * in the model's own buffers. */
+#define SOMHIP_NOALLOC_MAX_ROWS (1 << 21)
+int somhip_find_bmus_noalloc(somhip_model *model, const float *x, int64_t n, int precision, int arithmetic,
+                             int32_t *out);
+
 /* Activation map: result[bmu(x_i)] += rho_i for rho_i > 0 (rho == NULL means ones), accumulated
  * into the caller's device FP64 array (mn). Per call the sum is exact (fixed-point integers), so
  * the result does not depend on launch shape or batching.

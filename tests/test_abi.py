@@ -38,6 +38,8 @@ def test_header_constants_match_binding():
     assert int(consts["SOMHIP_OK"]) == _lib.OK
     assert int(consts["SOMHIP_ERR_ARG"]) == _lib.ERR_ARG
     assert int(consts["SOMHIP_ERR_NODEVICE"]) == _lib.ERR_NODEVICE
+    shift = re.search(r"#define\s+SOMHIP_NOALLOC_MAX_ROWS\s+\(1 << (\d+)\)", text)
+    assert shift and (1 << int(shift.group(1))) == _lib.NOALLOC_MAX_ROWS
     assert int(consts["SOMHIP_PREC_EXACT"]) == _lib.PREC_EXACT
     assert int(consts["SOMHIP_PREC_FP32"]) == _lib.PREC_FP32
     assert int(consts["SOMHIP_PREC_BF16_RERANK"]) == _lib.PREC_BF16_RERANK
