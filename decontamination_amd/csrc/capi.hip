@@ -625,7 +625,8 @@ int somhip_batch_epoch_accumulate(somhip_accum *A, const float *x, const double 
         (rc = W.d1.ensure(sizeof(double) * n)) || (rc = W.part.ensure(sizeof(double) * 3 * nparts)) ||
         (rc = W.bounds.ensure(512)) || (rc = W.shifts.ensure(512)) ||
         (rc = W.table.ensure((size_t)nwg * M->mn * nv * 8)) ||
-        (rc = W.xmax.ensure(sizeof(float) * (size_t)xrows * M->dpad)))
+        (rc = W.xmax.ensure(sizeof(float) * (size_t)xrows * M->dpad)) ||
+        (rc = W.bucket.ensure(sizeof(unsigned int) * (size_t)n)))
         return rc;
     int32_t *c1 = (int32_t *)W.c1.p, *c2 = (int32_t *)W.c2.p;
     double *d1 = (double *)W.d1.p;
@@ -652,7 +653,7 @@ int somhip_batch_epoch_accumulate(somhip_accum *A, const float *x, const double 
     }
     TimerScope ts(C, TK_SCATTER);
     return somhip_launch_scatter(C->stream, c1, x, M->dim, rho, 1, n, (const int *)W.shifts.p, M->mn, nv, nwg,
-                                 (unsigned long long *)W.table.p, A->acc, C->lds_per_block);
+                                 (unsigned long long *)W.table.p, A->acc, C->lds_per_block, (unsigned int *)W.bucket.p);
 }
 
 int somhip_batch_epoch_update(somhip_accum *A, int64_t cur_epoch, int64_t n_epochs, float sigma0, float *history_out)

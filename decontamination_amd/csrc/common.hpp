@@ -164,11 +164,11 @@ struct DevBuf {
 
 // Per-call device workspace (BMU outputs, reduction partials, fixed-point scales and tables).
 struct Workspace {
-    DevBuf c1, c2, d1, part, bounds, shifts, table, xmax, maps;
+    DevBuf c1, c2, d1, part, bounds, shifts, table, xmax, maps, bucket;
     void release()
     {
         c1.release(); c2.release(); d1.release(); part.release(); bounds.release(); shifts.release(); table.release();
-        xmax.release(); maps.release();
+        xmax.release(); maps.release(); bucket.release();
     }
 };
 

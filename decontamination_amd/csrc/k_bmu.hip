@@ -516,8 +516,10 @@ __device__ __forceinline__ void bmu_bf16_body(const BmuModelDev &Md, const float
         }
     };
     auto update = [&](int g, float t) {
+        // both as v_med3: a NaN t leaves the pair unchanged, as fminf would, without the
+        // canonicalising moves fminf needs in IEEE mode
         m2[g] = __builtin_amdgcn_fmed3f(m1[g], m2[g], t);
-        m1[g] = fminf(m1[g], t);
+        m1[g] = __builtin_amdgcn_fmed3f(m1[g], t, -__builtin_inff());
     };
     int cnt[G], pool_n[G];  // pool_n: wave-uniform fill of the group's shared pool
     bool lost[G];           // a candidate of this lane found both its list and the pool full (D = 32)
@@ -532,8 +534,8 @@ __device__ __forceinline__ void bmu_bf16_body(const BmuModelDev &Md, const float
     // the quads without a hit (hits in a block usually fall in one quad).
     // Rows of a quad with a hit cost one wave compare and one scalar test each when none of their lanes
     // hits. Past CAPL a lane's candidates go to its group's shared pool when there is one (slots handed
-    // out in lane order from the wave's ballot), else the lane is lost to the fallback. Lanes without a
-    // hit, and candidates past the list, write their own-list store to the spare last row.
+    // out in lane order from the wave's ballot), else the lane is lost to the fallback. Only the lanes
+    // with a hit store (an exec-masked store); candidates past the list land in the spare last row.
     auto record = [&](int blk, int g, const f32x16 &v, const float (&q)[4], float t) {
         if (__builtin_amdgcn_ballot_w64(t <= T[g]) != 0ull) {
             const int sbase_blk = blk * 32 + 4 * khalf;
@@ -546,9 +548,8 @@ __device__ __forceinline__ void bmu_bf16_body(const BmuModelDev &Md, const float
 #pragma unroll
                 for (int r = 4 * i; r < 4 * i + 4; r++) {
                     if (__builtin_expect(__builtin_amdgcn_ballot_w64(v[r] <= T[g]) != 0ull, 0)) {  // uniform, rare
-                        const bool hit = v[r] <= T[g];  // lanes without a hit write to the spare row
+                        const bool hit = v[r] <= T[g];
                         const unsigned short slot = (unsigned short)(sbase_blk + 8 * (r >> 2) + (r & 3));
-                        cw[g][hit ? min(cnt[g], CAPL) : CAPL][lane] = slot;
                         if constexpr (POOL > 0) {
                             const bool spill = hit && cnt[g] >= CAPL;
                             const unsigned long long sp = __builtin_amdgcn_ballot_w64(spill);
@@ -560,7 +561,10 @@ __device__ __forceinline__ void bmu_bf16_body(const BmuModelDev &Md, const float
                                 pool_n[g] += __builtin_popcountll(sp);
                             }
                         }
-                        cnt[g] += hit ? 1 : 0;
+                        if (hit) {
+                            cw[g][min(cnt[g], CAPL)][lane] = slot;
+                            cnt[g] += 1;
+                        }
                     }
                 }
             }
@@ -628,16 +632,31 @@ __device__ __forceinline__ void bmu_bf16_body(const BmuModelDev &Md, const float
                     wr[4 * q] = t[0]; wr[4 * q + 1] = t[1]; wr[4 * q + 2] = t[2]; wr[4 * q + 3] = t[3];
                 }
             };
-            float wn[DP];
-            int jn = -1;
-            if (mycnt > 0) load_row(0, wn, jn);
-            for (int e = 0; e < mycnt; e++) {
-                float wr[DP];
+            if constexpr (DP <= 16) {
+                float wn[DP];
+                int jn = -1;
+                if (mycnt > 0) load_row(0, wn, jn);
+                for (int e = 0; e < mycnt; e++) {
+                    float wr[DP];
 #pragma unroll
-                for (int k = 0; k < DP; k++) wr[k] = wn[k];
-                const int j = jn;
-                if (e + 1 < mycnt) load_row(e + 1, wn, jn);
-                if (j >= 0) tp.push(sqd_mode<MODE>(wr, xr, DP), j);  // zero-padded dims add exact zeros
+                    for (int k = 0; k < DP; k++) wr[k] = wn[k];
+                    const int j = jn;
+                    if (e + 1 < mycnt) load_row(e + 1, wn, jn);
+                    if (j >= 0) tp.push(sqd_mode<MODE>(wr, xr, DP), j);  // zero-padded dims add exact zeros
+                }
+            } else {
+                // wider rows: one row in registers at a time, so the rerank does not set the kernel's
+                // register peak (which would halve the waves per SIMD)
+                for (int e = 0; e < mycnt; e++) {
+                    float wr[DP];
+                    int j;
+                    load_row(e, wr, j);
+                    // keeps the FP64 arithmetic from holding the sample converted to double across the
+                    // loop (64 more registers); the empty statement emits no instruction
+#pragma unroll
+                    for (int k = 0; k < DP; k++) asm volatile("" : "+v"(xr[k]));
+                    if (j >= 0) tp.push(sqd_mode<MODE>(wr, xr, DP), j);
+                }
             }
             // the group's spilled candidates: each entry is reranked by the lane that recorded it
             const int np = POOL > 0 ? min(pool_n[g], POOL) : 0;
@@ -652,6 +671,10 @@ __device__ __forceinline__ void bmu_bf16_body(const BmuModelDev &Md, const float
                     for (int q = 0; q < DP / 4; q++) {
                         const f32x4 t = row[q];
                         wr[4 * q] = t[0]; wr[4 * q + 1] = t[1]; wr[4 * q + 2] = t[2]; wr[4 * q + 3] = t[3];
+                    }
+                    if constexpr (DP > 16) {
+#pragma unroll
+                        for (int k = 0; k < DP; k++) asm volatile("" : "+v"(xr[k]));
                     }
                     if (j >= 0) tp.push(sqd_mode<MODE>(wr, xr, DP), j);
                 }
@@ -678,7 +701,7 @@ __device__ __forceinline__ void bmu_bf16_body(const BmuModelDev &Md, const float
 }
 
 template <int DP, int G, int MODE, bool NEED_C2>
-__global__ __launch_bounds__(NW_BF16 * WAVE) void bmu_bf16_kernel(BmuModelDev Md, const float *__restrict__ x,
+__global__ __launch_bounds__(NW_BF16 * WAVE, 4) void bmu_bf16_kernel(BmuModelDev Md, const float *__restrict__ x,
                                                                  int64_t n, int default_index,
                                                                  int32_t *__restrict__ c1, int32_t *__restrict__ c2,
                                                                  double *__restrict__ d1,

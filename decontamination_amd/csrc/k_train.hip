@@ -246,6 +246,100 @@ __global__ __launch_bounds__(SCATTER_NT) void scatter_fixed_kernel(const int32_t
     }
 }
 
+// Bucketed form for maps that need several neuron slices. Workgroup r first sweeps its range's BMU
+// indices once: an LDS histogram per slice (slice = bmu >> log2_slice), a prefix over the slices, and
+// every sample's entry ((sample - i0) << 12 | neuron-in-slice) written to bucket[i0 + position], grouped
+// by slice. Each slice pass then reads only its own bucket entries (one per thread, the sample row as
+// 16-byte vectors when XVEC) instead of sweeping every index of the range again. Positions within a
+// bucket depend on atomic order, but the sums are integers, so the tables do not.
+constexpr int BUCKET_MAX_SLICES = 1024;
+constexpr int BUCKET_LOCAL_BITS = 20;  // samples per workgroup range < 2^20
+constexpr int BUCKET_NEURON_BITS = 12; // neurons per slice <= 2^12
+
+template <int DP, bool XVEC>
+__global__ __launch_bounds__(SCATTER_NT) void scatter_bucket_kernel(const int32_t *__restrict__ bmu,
+                                                                    const float *__restrict__ x, int dim,
+                                                                    const double *__restrict__ rho, int rho_f32,
+                                                                    int64_t n, int64_t per_wg,
+                                                                    const int *__restrict__ shifts, int mn, int nv,
+                                                                    int log2_slice, int nslices,
+                                                                    unsigned int *__restrict__ bucket,
+                                                                    unsigned long long *__restrict__ partial)
+{
+    extern __shared__ unsigned long long lds[];  // [slice_len][nv]
+    __shared__ unsigned int start[BUCKET_MAX_SLICES + 1];
+    __shared__ unsigned int cursor[BUCKET_MAX_SLICES];
+    const int slice_len = 1 << log2_slice;
+    const int64_t i0 = (int64_t)blockIdx.x * per_wg;
+    const int64_t i1 = min(n, i0 + per_wg);
+    unsigned long long *out = partial + (int64_t)blockIdx.x * mn * nv;
+    unsigned int *bk = bucket + i0;
+
+    for (int s = threadIdx.x; s < nslices; s += SCATTER_NT) cursor[s] = 0u;
+    __syncthreads();
+    for (int64_t i = i0 + threadIdx.x; i < i1; i += SCATTER_NT) {
+        const int c = bmu[i];
+        if (c >= 0 && c < mn) atomicAdd(&cursor[c >> log2_slice], 1u);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned int acc = 0u;
+        for (int s = 0; s < nslices; s++) {
+            start[s] = acc;
+            acc += cursor[s];
+            cursor[s] = start[s];
+        }
+        start[nslices] = acc;
+    }
+    __syncthreads();
+    for (int64_t i = i0 + threadIdx.x; i < i1; i += SCATTER_NT) {
+        const int c = bmu[i];
+        if (c >= 0 && c < mn) {
+            const unsigned int pos = atomicAdd(&cursor[c >> log2_slice], 1u);
+            bk[pos] = ((unsigned int)(i - i0) << BUCKET_NEURON_BITS) | (unsigned int)(c & (slice_len - 1));
+        }
+    }
+    __syncthreads();  // the bucket entries written above are read back by other threads of the workgroup
+
+    for (int s = 0; s < nslices; s++) {
+        const int c0 = s << log2_slice;
+        const int cl = min(slice_len, mn - c0);
+        for (int e = threadIdx.x; e < cl * nv; e += SCATTER_NT) lds[e] = 0ull;
+        __syncthreads();
+        for (unsigned int e = start[s] + threadIdx.x; e < start[s + 1]; e += SCATTER_NT) {
+            const unsigned int ent = bk[e];
+            const int64_t i = i0 + (ent >> BUCKET_NEURON_BITS);
+            unsigned long long *row = lds + (int64_t)(ent & ((1u << BUCKET_NEURON_BITS) - 1u)) * nv;
+            double r = rho ? rho[i] : 1.0;
+            if (rho_f32) r = (double)(float)r;
+            if (!(r > 0.0 && r < 1e300)) continue;
+            if (x) {
+                float xv[DP];
+                if (XVEC) {
+                    const f32x4 *xr = (const f32x4 *)(x + i * dim);
+#pragma unroll
+                    for (int v = 0; v < DP / 4; v++) {
+                        const f32x4 t = 4 * v < dim ? xr[v] : f32x4{0.f, 0.f, 0.f, 0.f};
+                        xv[4 * v] = t[0]; xv[4 * v + 1] = t[1]; xv[4 * v + 2] = t[2]; xv[4 * v + 3] = t[3];
+                    }
+                } else {
+#pragma unroll
+                    for (int k = 0; k < DP; k++) xv[k] = k < dim ? x[i * dim + k] : 0.f;
+                }
+#pragma unroll
+                for (int k = 0; k < DP; k++) {
+                    // r * x is exact in FP64 (24-bit mantissas), and so is the power-of-two scaling
+                    if (k < dim) atomicAdd(&row[k], (unsigned long long)llrint(ldexp(r * (double)xv[k], shifts[k])));
+                }
+            }
+            atomicAdd(&row[nv - 1], (unsigned long long)llrint(ldexp(r, shifts[nv - 1])));
+        }
+        __syncthreads();
+        for (int e = threadIdx.x; e < cl * nv; e += SCATTER_NT) out[(int64_t)c0 * nv + e] = lds[e];
+        __syncthreads();
+    }
+}
+
 // acc[e] += 2^-shift * sum_b partial[b][e] (integer sum, exact)
 __global__ void scatter_reduce_kernel(const unsigned long long *__restrict__ partial, int nwg, int64_t width, int nv,
                                       const int *__restrict__ shifts, double *__restrict__ acc)
@@ -575,7 +669,7 @@ int somhip_scatter_nwg(int64_t n, int mn, int nv, int num_cus, size_t table_budg
 
 int somhip_launch_scatter(hipStream_t st, const int32_t *bmu, const float *x, int dim, const double *rho, int rho_f32,
                           int64_t n, const int *shifts, int mn, int nv, int nwg, unsigned long long *partial,
-                          double *acc, size_t lds_limit)
+                          double *acc, size_t lds_limit, unsigned int *bucket)
 {
     if (n <= 0) return SOMHIP_OK;
     constexpr size_t queue_bytes = (size_t)(SCATTER_NT / 64) * 128 * (4 + 2);
@@ -585,8 +679,8 @@ int somhip_launch_scatter(hipStream_t st, const int32_t *bmu, const float *x, in
         somhip_set_error("LDS too small for one neuron row");
         return SOMHIP_ERR_UNSUPPORTED;
     }
+    const bool several_slices = slice_len < mn;
     if (slice_len > mn) slice_len = mn;
-    const size_t lds = (size_t)slice_len * nv * 8 + queue_bytes;
     const int64_t per_wg = (n + nwg - 1) / nwg;
     if (per_wg > 0xffffffffll) {
         somhip_set_error("too many samples per scatter workgroup");
@@ -594,6 +688,30 @@ int somhip_launch_scatter(hipStream_t st, const int32_t *bmu, const float *x, in
     }
     const int dpad = (dim + 15) / 16 * 16;
     const bool xvec = x && dim % 4 == 0 && ((uintptr_t)x & 15) == 0;
+    // several slices: the bucketed kernel, with a power-of-two slice that fits beside its static arrays
+    int log2_slice = 0, nslices = 0;
+    if (bucket && several_slices && per_wg < (1ll << BUCKET_LOCAL_BITS)) {
+        const size_t avail = std::min(LDS_SCATTER_MAX, lds_limit) - sizeof(unsigned int) * (2 * BUCKET_MAX_SLICES + 1);
+        while (log2_slice < BUCKET_NEURON_BITS && ((size_t)2 << log2_slice) * nv * 8 <= avail) log2_slice++;
+        nslices = (mn + (1 << log2_slice) - 1) >> log2_slice;
+        if (((size_t)1 << log2_slice) * nv * 8 > avail || nslices > BUCKET_MAX_SLICES) nslices = 0;
+    }
+    if (nslices > 0) {
+        const size_t lds = ((size_t)1 << log2_slice) * nv * 8;
+#define SOM_SB(D)                                                                                              \
+    do {                                                                                                       \
+        auto kern = xvec ? scatter_bucket_kernel<D, true> : scatter_bucket_kernel<D, false>;                   \
+        SOM_HIP_TRY(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds)); \
+        hipLaunchKernelGGL(kern, dim3(nwg), dim3(SCATTER_NT), lds, st, bmu, x, dim, rho, rho_f32, n, per_wg,     \
+                           shifts, mn, nv, log2_slice, nslices, bucket, partial);                               \
+    } while (0)
+        if (!x || dpad <= 16) SOM_SB(16);
+        else if (dpad == 32) SOM_SB(32);
+        else if (dpad == 48) SOM_SB(48);
+        else SOM_SB(64);
+#undef SOM_SB
+    } else {
+        const size_t lds = (size_t)slice_len * nv * 8 + queue_bytes;
 #define SOM_SC(D)                                                                                              \
     do {                                                                                                       \
         auto kern = xvec ? scatter_fixed_kernel<D, true> : scatter_fixed_kernel<D, false>;                     \
@@ -601,11 +719,12 @@ int somhip_launch_scatter(hipStream_t st, const int32_t *bmu, const float *x, in
         hipLaunchKernelGGL(kern, dim3(nwg), dim3(SCATTER_NT), lds, st, bmu, x, dim, rho, rho_f32, n, per_wg,     \
                            shifts, mn, nv, slice_len, partial);                                                \
     } while (0)
-    if (!x || dpad <= 16) SOM_SC(16);
-    else if (dpad == 32) SOM_SC(32);
-    else if (dpad == 48) SOM_SC(48);
-    else SOM_SC(64);
+        if (!x || dpad <= 16) SOM_SC(16);
+        else if (dpad == 32) SOM_SC(32);
+        else if (dpad == 48) SOM_SC(48);
+        else SOM_SC(64);
 #undef SOM_SC
+    }
     SOM_HIP_TRY(hipGetLastError());
     const int64_t width = (int64_t)mn * nv;
     hipLaunchKernelGGL(scatter_reduce_kernel, dim3((unsigned)((width + 255) / 256)), dim3(256), 0, st, partial, nwg,

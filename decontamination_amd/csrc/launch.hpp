@@ -14,9 +14,11 @@ int somhip_launch_shifts(hipStream_t st, const unsigned int *bounds, int dim, in
 int somhip_launch_shifts_parts(hipStream_t st, const float *parts, int64_t rows, int dpad, int dim,
                                const unsigned int *rho_bound, int64_t n, int *shifts);
 int somhip_scatter_nwg(int64_t n, int mn, int nv, int num_cus, size_t table_budget);
+// bucket (nullable): n uint32 of workspace; with it, maps that need several neuron slices use the
+// bucketed kernel (each slice pass reads only its own samples' entries)
 int somhip_launch_scatter(hipStream_t st, const int32_t *bmu, const float *x, int dim, const double *rho, int rho_f32,
                           int64_t n, const int *shifts, int mn, int nv, int nwg, unsigned long long *partial,
-                          double *acc, size_t lds_limit);
+                          double *acc, size_t lds_limit, unsigned int *bucket = nullptr);
 int somhip_update_nsplit(int mn);
 int somhip_launch_update(hipStream_t st, const double *S, const float *topo, int mn, int dim, double sigma,
                          double *part, int nsplit, float *w, float *hist);

@@ -238,6 +238,34 @@ def test_one_step_vs_fp64_factorised(precision):
     assert rel.max() <= 1e-6, rel.max()
 
 
+@pytest.mark.parametrize("shape", [(64, 64, 16, "hexagonal"), (100, 100, 32, "hexagonal"), (90, 70, 20, "square")])
+def test_one_step_multi_slice_scatter_vs_fp64(shape):
+    """Maps whose per-neuron table needs several LDS slices (the bucketed scatter): one weighted epoch
+    with invalid rows and zero / negative / infinite weights, against the FP64 factorised oracle on the
+    FP64 oracle's BMUs."""
+    m, n, dim, topo = shape
+    print(f"multi-slice scatter {m}x{n}/D{dim}", flush=True)
+    N = 150_000
+    rng = np.random.default_rng(71 + dim)
+    X = rng.standard_normal((N, dim)).astype(np.float32)
+    X[11::4999] = np.nan
+    rho = rng.random(N) * 3.0
+    rho[::97] = 0.0
+    rho[1::101] = -1.0
+    rho[2::103] = np.inf
+    W0 = (rng.standard_normal((m * n, dim)) * 0.5).astype(np.float32)
+    som = dam.SOM_Batch(m, n, dim, topology=topo, arithmetic="fp64")
+    som._weights[...] = W0
+    som.train(X, rho, n_epochs=1, use_best_epoch=False)
+    sigma = R.sigma_at(max(m, n) / 2.0, 0, 1)
+    ok = np.isfinite(X).all(1) & (rho > 0) & np.isfinite(rho)
+    Xo, ro = X[ok], rho[ok].astype(np.float32).astype(np.float64)  # the density is rounded to the dtype
+    c1, _, _ = R.top2_omp(W0, Xo, FP64)
+    Wf, _, _, _ = R.factorised_epoch_fp64(W0, R.topography(m, n, topo), Xo, ro, sigma, bmu=c1, out_dtype=np.float64)
+    rel = np.abs(som.weights - Wf) / np.maximum(np.abs(Wf), 1e-3)
+    assert rel.max() <= 1e-6, rel.max()
+
+
 @pytest.mark.parametrize("shape", [(64, 64, 16, "hexagonal"), (32, 40, 32, "square")])
 def test_trained_map_bf16_equals_exact(shape):
     """On a map with trained structure (near-equal neighbour distances, the hard case for the BF16
