@@ -28,6 +28,23 @@ constexpr int WAVE = 64;
 constexpr int BMU_WAVES = 4;                 // waves per workgroup of the MFMA kernels
 constexpr float U32 = 5.9604644775390625e-8f;  // 2^-24
 
+// Pointers in the global address space. The BF16 body reads everything through these, so its accesses
+// are global loads and stores also when its pointers come from descriptors in memory (the serving form),
+// which the compiler would otherwise treat as generic and access with flat instructions.
+template <class T>
+using gptr = T __attribute__((address_space(1))) *;
+
+// p itself, or p in the global address space (GC); kernels whose pointers are kernel arguments keep
+// them as they are, since the compiler already knows where those point and assigns their registers best
+template <bool GC, class T>
+__device__ __forceinline__ auto aptr(T *p)
+{
+    if constexpr (GC) return (gptr<T>)p;
+    else return p;
+}
+__device__ __forceinline__ const f32x4 *as_f32x4(const float *p) { return (const f32x4 *)p; }
+__device__ __forceinline__ gptr<const f32x4> as_f32x4(gptr<const float> p) { return (gptr<const f32x4>)p; }
+
 // Each lane needs its partner (lane ^ 32) value: both hold the same sample column.
 __device__ __forceinline__ float xchg32(float v) { return __shfl_xor(v, 32, WAVE); }
 __device__ __forceinline__ int xchg32i(int v) { return __shfl_xor(v, 32, WAVE); }
@@ -120,9 +137,8 @@ struct SampleLoad {
     float ax[KC][8];  // |x| of the same dims (0 when not finite), for the fixed-point scales
 };
 
-template <int KC>
-__device__ __forceinline__ void load_sample(const float *__restrict__ x, int64_t n, int dim, int64_t s, int lane,
-                                            const float *__restrict__ centre, SampleLoad<KC> &L)
+template <int KC, class XP, class CP>
+__device__ __forceinline__ void load_sample(XP x, int64_t n, int dim, int64_t s, int lane, CP centre, SampleLoad<KC> &L)
 {
     const int hb = 8 * (lane >> 5);
 #pragma unroll
@@ -328,11 +344,11 @@ struct Bf16Smem {
     unsigned int pool[NW_BF16][G_BF16][Bf16Cand<KC>::POOL > 0 ? Bf16Cand<KC>::POOL : 1];  // (lane << 16) | slot
 };
 
-template <int DP, int G, int MODE, bool NEED_C2>
-__device__ __forceinline__ void bmu_bf16_body(const BmuModelDev &Md, const float *__restrict__ x, int64_t n,
-                                              int64_t tile_row0, int default_index, int32_t *__restrict__ c1,
-                                              int32_t *__restrict__ c2, double *__restrict__ d1,
-                                              int64_t *__restrict__ fb_list, unsigned int *__restrict__ fb_count,
+template <int DP, int G, int MODE, bool NEED_C2, bool GC>
+__device__ __forceinline__ void bmu_bf16_body(const BmuModelDev &Md, const float *__restrict__ x_in, int64_t n,
+                                              int64_t tile_row0, int default_index, int32_t *__restrict__ c1_out,
+                                              int32_t *__restrict__ c2_out, double *__restrict__ d1_out,
+                                              int64_t *__restrict__ fb_list_out, unsigned int *__restrict__ fb_count,
                                               int64_t fb_tag, Bf16Smem<DP / 16> &sm, float *__restrict__ xmax_part)
 {
     constexpr int KC = DP / 16;
@@ -343,13 +359,18 @@ __device__ __forceinline__ void bmu_bf16_body(const BmuModelDev &Md, const float
     constexpr int NT = NW_BF16 * WAVE;   // threads per workgroup
     constexpr int FU = FRAG_UNITS / NT;  // fragment units per thread
     static_assert(FRAG_UNITS % NT == 0 && NORM_UNITS <= NT, "chunk layout");
-    const float *__restrict__ centre = Md.centre;
-    const float *__restrict__ wmax = Md.wmax;
-    const float *__restrict__ scal = Md.scal;
-    const uint4 *__restrict__ gfrag = (const uint4 *)Md.frag;
-    const uint4 *__restrict__ gnorm = (const uint4 *)Md.norms4;
-    const int *__restrict__ perm = Md.perm;
-    const float *__restrict__ wslot = Md.wslot;
+    const auto centre = aptr<GC>(Md.centre);
+    const auto wmax = aptr<GC>(Md.wmax);
+    const auto scal = aptr<GC>(Md.scal);
+    const auto gfrag = aptr<GC>((const uint4 *)Md.frag);
+    const auto gnorm = aptr<GC>((const uint4 *)Md.norms4);
+    const auto perm = aptr<GC>(Md.perm);
+    const auto wslot = aptr<GC>(Md.wslot);
+    const auto x = aptr<GC>(x_in);
+    const auto c1 = aptr<GC>(c1_out);
+    const auto c2 = aptr<GC>(c2_out);
+    const auto d1 = aptr<GC>(d1_out);
+    const auto fb_list = aptr<GC>(fb_list_out);
     const int nblk = Md.nblk, dim = Md.dim;
     const int tid = threadIdx.x;
     const int lane = tid & (WAVE - 1);
@@ -363,6 +384,10 @@ __device__ __forceinline__ void bmu_bf16_body(const BmuModelDev &Md, const float
     const int64_t norm_units_total = (int64_t)nblk * 8;
 
     const float nwmax = scal[0];
+    // -inf as a value the compiler cannot see, so v_med3(m1, t, -inf) is not folded back into a minimum
+    // that needs canonicalising moves in IEEE mode
+    float neg_inf = -__builtin_inff();
+    asm volatile("" : "+v"(neg_inf));
     bf16x8 bh[G][KC], bl[G][KC];
     float E[G], K[G];
     float amax[KC][8];
@@ -416,7 +441,7 @@ __device__ __forceinline__ void bmu_bf16_body(const BmuModelDev &Md, const float
 #pragma unroll
         for (int u = 0; u < FU; u++) {
             const int64_t idx = fb + tid + (int64_t)u * NT;
-            pf[u] = idx < frag_units_total ? gfrag[idx] : make_uint4(0u, 0u, 0u, 0u);
+            pf[u] = idx < frag_units_total ? (uint4)gfrag[idx] : make_uint4(0u, 0u, 0u, 0u);
         }
         const int64_t nidx = (int64_t)c * NORM_UNITS + tid;
         if (tid < NORM_UNITS && nidx < norm_units_total) pn = gnorm[nidx];
@@ -519,7 +544,7 @@ __device__ __forceinline__ void bmu_bf16_body(const BmuModelDev &Md, const float
         // both as v_med3: a NaN t leaves the pair unchanged, as fminf would, without the
         // canonicalising moves fminf needs in IEEE mode
         m2[g] = __builtin_amdgcn_fmed3f(m1[g], m2[g], t);
-        m1[g] = __builtin_amdgcn_fmed3f(m1[g], t, -__builtin_inff());
+        m1[g] = __builtin_amdgcn_fmed3f(m1[g], t, neg_inf);
     };
     int cnt[G], pool_n[G];  // pool_n: wave-uniform fill of the group's shared pool
     bool lost[G];           // a candidate of this lane found both its list and the pool full (D = 32)
@@ -625,7 +650,7 @@ __device__ __forceinline__ void bmu_bf16_body(const BmuModelDev &Md, const float
             auto load_row = [&](int e, float (&wr)[DP], int &j) {
                 const int slot = cw[g][e][lane];
                 j = perm[slot];
-                const f32x4 *row = (const f32x4 *)(wslot + (int64_t)slot * DP);
+                const auto row = as_f32x4(wslot + (int64_t)slot * DP);
 #pragma unroll
                 for (int q = 0; q < DP / 4; q++) {
                     const f32x4 t = row[q];
@@ -666,7 +691,7 @@ __device__ __forceinline__ void bmu_bf16_body(const BmuModelDev &Md, const float
                     float wr[DP];
                     const int slot = (int)(ent & 0xFFFFu);
                     const int j = perm[slot];
-                    const f32x4 *row = (const f32x4 *)(wslot + (int64_t)slot * DP);
+                    const auto row = as_f32x4(wslot + (int64_t)slot * DP);
 #pragma unroll
                     for (int q = 0; q < DP / 4; q++) {
                         const f32x4 t = row[q];
@@ -710,13 +735,13 @@ __global__ __launch_bounds__(NW_BF16 * WAVE, 4) void bmu_bf16_kernel(BmuModelDev
                                                                  float *__restrict__ xmax_part)
 {
     __shared__ Bf16Smem<DP / 16> sm;
-    bmu_bf16_body<DP, G, MODE, NEED_C2>(Md, x, n, (int64_t)blockIdx.x * SOM_TILE_ROWS, default_index, c1, c2, d1,
-                                        fb_list, fb_count, 0, sm, xmax_part);
+    bmu_bf16_body<DP, G, MODE, NEED_C2, false>(Md, x, n, (int64_t)blockIdx.x * SOM_TILE_ROWS, default_index, c1, c2,
+                                               d1, fb_list, fb_count, 0, sm, xmax_part);
 }
 
 // Serving form: one workgroup per tile; tiles of different requests and models share one launch.
 template <int DP, int G, int MODE, bool NEED_C2>
-__global__ __launch_bounds__(NW_BF16 * WAVE) void bmu_bf16_tiles_kernel(const BmuModelDev *__restrict__ models,
+__global__ __launch_bounds__(NW_BF16 * WAVE, 4) void bmu_bf16_tiles_kernel(const BmuModelDev *__restrict__ models,
                                                                        const BmuTile *__restrict__ tiles,
                                                                        int64_t *__restrict__ fb_list,
                                                                        unsigned int *__restrict__ fb_count)
@@ -725,8 +750,9 @@ __global__ __launch_bounds__(NW_BF16 * WAVE) void bmu_bf16_tiles_kernel(const Bm
     const BmuTile T = tiles[blockIdx.x];
     if (T.n <= 0) return;  // uniform over the workgroup
     const BmuModelDev Md = models[T.model];
-    bmu_bf16_body<DP, G, MODE, NEED_C2>(Md, T.x, T.n, T.row0, T.default_index, T.c1, T.c2, T.d1, fb_list, fb_count,
-                                        (int64_t)blockIdx.x << FB_TILE_SHIFT, sm, nullptr);
+    // the descriptors' pointers are generic to the compiler: global_* accesses instead of flat ones
+    bmu_bf16_body<DP, G, MODE, NEED_C2, true>(Md, T.x, T.n, T.row0, T.default_index, T.c1, T.c2, T.d1, fb_list,
+                                              fb_count, (int64_t)blockIdx.x << FB_TILE_SHIFT, sm, nullptr);
 }
 
 // Exact top-2 of one sample by one workgroup of FB_THREADS threads: thread t scans the permuted
@@ -748,21 +774,24 @@ __device__ __forceinline__ Top2 group_exact_top2(const float *__restrict__ xrow,
     const int tid = threadIdx.x;
     float xr[DP];
 #pragma unroll
-    for (int k = 0; k < DP; k++) xr[k] = k < dim ? xrow[k] : 0.f;
+    for (int k = 0; k < DP; k++) xr[k] = k < dim ? ((gptr<const float>)xrow)[k] : 0.f;
     Top2 tp;
     tp.init();
     const int nslots = Md.nblk * 32;
-    const f32x4 *__restrict__ rows = (const f32x4 *)Md.wslot;
+    const gptr<const f32x4> rows = (gptr<const f32x4>)Md.wslot;
+    const gptr<const int> perm = (gptr<const int>)Md.perm;
     for (int p0 = tid; p0 < nslots; p0 += U * FB_THREADS) {
         float wr[U][DP];
         int jj[U];
 #pragma unroll
         for (int u = 0; u < U; u++) {
             const int p = p0 + u * FB_THREADS;
-            jj[u] = p < nslots ? Md.perm[p] : -1;
+            jj[u] = -1;
+            if (p < nslots) jj[u] = perm[p];
 #pragma unroll
             for (int q = 0; q < DP / 4; q++) {
-                const f32x4 t = p < nslots ? rows[(int64_t)p * (DP / 4) + q] : f32x4{0.f, 0.f, 0.f, 0.f};
+                f32x4 t = f32x4{0.f, 0.f, 0.f, 0.f};
+                if (p < nslots) t = rows[(int64_t)p * (DP / 4) + q];
                 wr[u][4 * q] = t[0]; wr[u][4 * q + 1] = t[1]; wr[u][4 * q + 2] = t[2]; wr[u][4 * q + 3] = t[3];
             }
         }
@@ -828,9 +857,9 @@ __global__ __launch_bounds__(FB_THREADS) void bmu_exact_tiles_kernel(const BmuMo
         const BmuModelDev Md = models[T.model];
         const Top2 tp = group_exact_top2<DP, MODE>(T.x + i * Md.dim, Md.dim, Md, red);
         if (threadIdx.x == 0) {
-            T.c1[i] = tp.i1 >= 0 ? tp.i1 : T.default_index;
-            if (T.c2) T.c2[i] = tp.i2;
-            if (T.d1) T.d1[i] = tp.i1 >= 0 ? tp.d1 : __builtin_nan("");
+            ((gptr<int32_t>)T.c1)[i] = tp.i1 >= 0 ? tp.i1 : T.default_index;
+            if (T.c2) ((gptr<int32_t>)T.c2)[i] = tp.i2;
+            if (T.d1) ((gptr<double>)T.d1)[i] = tp.i1 >= 0 ? tp.d1 : __builtin_nan("");
         }
     }
 }
