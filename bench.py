@@ -783,6 +783,9 @@ def main():
                             (1 << 18, 1 << 16, 1 << 15), "c3")
         w3["x_dev"].free()
         r3 = w3["res"]
+        if r3[head_name].get("roofline"):
+            r3[head_name]["roofline"]["mfma_busy"] = profile_figure("som_epoch_100x100_d32_counters.md", "mfma_busy",
+                                                                    "bmu_bf16_kernel")
         out["c3"] = {"workload": f"batch SOM epoch, 100x100 hexagonal, dim 32, N={X3.shape[0]} (HEALPix nside 2048, "
                                  "16 lognormal + 14 Gaussian + 2 angular), sigma0=50, timed epochs of the 10-epoch "
                                  "schedule from init_rand(0), X resident in HBM",
