@@ -338,7 +338,7 @@ template <int KC>
 struct Bf16Smem {
     static constexpr int CB = CHUNK_KB / KC;  // blocks per chunk
     static constexpr int CAPL = Bf16Cand<KC>::CAPL;
-    uint4 frag[2][FRAG_UNITS];          // [buf][cb][kc][hi,lo][lane] 16 B
+    u32x4 frag[2][FRAG_UNITS];          // [buf][cb][kc][hi,lo][lane] 16 B
     float norms[2][CB * 32];            // [buf][cb][row]
     unsigned short cand[NW_BF16][G_BF16][CAPL + 1][WAVE];  // last row: writes past a full list
     unsigned int pool[NW_BF16][G_BF16][Bf16Cand<KC>::POOL > 0 ? Bf16Cand<KC>::POOL : 1];  // (lane << 16) | slot
@@ -362,8 +362,10 @@ __device__ __forceinline__ void bmu_bf16_body(const BmuModelDev &Md, const float
     const auto centre = aptr<GC>(Md.centre);
     const auto wmax = aptr<GC>(Md.wmax);
     const auto scal = aptr<GC>(Md.scal);
-    const auto gfrag = aptr<GC>((const uint4 *)Md.frag);
-    const auto gnorm = aptr<GC>((const uint4 *)Md.norms4);
+    // loads through a plain vector type: a class type (uint4) cannot be copied from a global-address-space
+    // reference in the host compilation pass
+    const auto gfrag = aptr<GC>((const u32x4 *)Md.frag);
+    const auto gnorm = aptr<GC>((const u32x4 *)Md.norms4);
     const auto perm = aptr<GC>(Md.perm);
     const auto wslot = aptr<GC>(Md.wslot);
     const auto x = aptr<GC>(x_in);
@@ -434,14 +436,14 @@ __device__ __forceinline__ void bmu_bf16_body(const BmuModelDev &Md, const float
     if (xmax_part) wave_dim_max<KC>(amax, lane, xmax_part);
 
     // chunk streaming ---------------------------------------------------------------------------
-    uint4 pf[FU];
-    uint4 pn = make_uint4(0u, 0u, 0u, 0u);
+    u32x4 pf[FU];
+    u32x4 pn = {0u, 0u, 0u, 0u};
     auto load_chunk = [&](int c) {
         const int64_t fb = (int64_t)c * FRAG_UNITS;
 #pragma unroll
         for (int u = 0; u < FU; u++) {
             const int64_t idx = fb + tid + (int64_t)u * NT;
-            pf[u] = idx < frag_units_total ? (uint4)gfrag[idx] : make_uint4(0u, 0u, 0u, 0u);
+            pf[u] = idx < frag_units_total ? gfrag[idx] : u32x4{0u, 0u, 0u, 0u};
         }
         const int64_t nidx = (int64_t)c * NORM_UNITS + tid;
         if (tid < NORM_UNITS && nidx < norm_units_total) pn = gnorm[nidx];
@@ -449,7 +451,7 @@ __device__ __forceinline__ void bmu_bf16_body(const BmuModelDev &Md, const float
     auto store_chunk = [&](int buf) {
 #pragma unroll
         for (int u = 0; u < FU; u++) sm.frag[buf][tid + u * NT] = pf[u];
-        if (tid < NORM_UNITS) ((uint4 *)sm.norms[buf])[tid] = pn;
+        if (tid < NORM_UNITS) ((u32x4 *)sm.norms[buf])[tid] = pn;
     };
     auto read_block = [&](int buf, int cb, bf16x8 (&ah)[KC], bf16x8 (&al)[KC], f32x16 &cin) {
 #pragma unroll
