@@ -12,6 +12,12 @@ Parity status of each oracle function (what pins it):
                                  the global mean), chunking invariance.
   train iter step                golden vector-count centroids (test_som_batch.py:63-83, rtol 1e-4).
   factorised FP64 epoch / top2   agreement with the C FP64 kernels on random inputs.
+  top2_omp (parallel top-2, d1)  the sequential top2 above (ties, NaN rows), d1 against the mode's distance
+                                 formula, golden winners.
+  activation_map_omp             golden activation map at 1, 3 and 8 threads; the sequential map with
+                                 real weights.
+  trajectory_fp64                golden epoch centroids (one epoch); three epochs against the sequential
+                                 FP64 restatement of SOM_Batch.train.
   HEALPix pixel-centre z         the reference's own local-coordinate formula restated independently
                                  (hp/__init__.py:376-419 with u = v = 0.5), equal-area symmetry.
 QE/TE values on hexagonal maps have no reference number (the reference's test asserts nothing,
@@ -236,3 +242,61 @@ def test_healpix_pixel_centre_z(nside):
     # equal-area pixels: z is symmetric about the equator and uniformly spread
     assert abs(z.sum()) < 1e-9 * len(z)
     assert np.all(np.abs(z) < 1.0)
+
+
+@pytest.mark.parametrize("mode", [REF32, FP64])
+def test_top2_omp_matches_sequential_top2(mode):
+    """orc_top2_omp (parallel over samples) against the sequential orc_top2 pinned above, duplicate
+    neurons (lowest index wins) and a NaN row included; d1 against the distance formula of the mode."""
+    rng = np.random.default_rng(21)
+    W = rng.standard_normal((30, 5)).astype(np.float32)
+    W[7] = W[3]
+    X = rng.standard_normal((2000, 5)).astype(np.float32)
+    X[5] = W[3]
+    X[11, 2] = np.nan
+    topo = R.topography(5, 6)
+    c1, c2, d1 = R.top2_omp(W, X, mode)
+    g1, g2 = R.top2(W, topo, X, None, mode)
+    ok = ~np.isnan(X).any(1)
+    assert np.array_equal(c1[ok], g1[ok]) and np.array_equal(c2[ok], g2[ok])
+    assert c1[5] == 3 and c2[5] == 7
+    if mode == FP64:
+        ref = ((X[ok].astype(np.float64) - W[c1[ok]].astype(np.float64)) ** 2).sum(1)
+    else:
+        diff = (X[ok] - W[c1[ok]]).astype(np.float32)
+        ref = (diff * diff).astype(np.float32).astype(np.float64).sum(1)
+    assert np.allclose(d1[ok], ref, rtol=1e-12, atol=0)
+    assert np.array_equal(R.top2_omp(R.init_rand(4, 4, 4, seed=0), golden_data(8), mode)[0],
+                          np.array(GOLD["winners"]))
+
+
+@pytest.mark.parametrize("threads", [1, 3, 8])
+def test_activation_map_omp_matches_sequential(threads):
+    W0 = R.init_rand(4, 4, 4, seed=0)
+    act = R.activation_map_omp(W0, golden_data(25_000), threads=threads)
+    assert np.array_equal(act.reshape(4, 4), np.array(GOLD["activation_map"], dtype=np.float64))
+    rng = np.random.default_rng(22)
+    X = rng.standard_normal((3000, 6)).astype(np.float32)
+    W = rng.standard_normal((12, 6)).astype(np.float32)
+    rho = rng.random(3000)
+    for mode in (REF32, FP64):
+        assert np.allclose(R.activation_map_omp(W, X, rho, mode, threads), R.activation_map(W, X, rho, mode),
+                           rtol=1e-12, atol=0)
+
+
+def test_trajectory_fp64_golden_and_multi_epoch():
+    """One epoch from init_rand(0) on the reference's 4x4 square test reproduces its golden centroids
+    (test_som_batch.py:30-50, rtol 1e-4); three epochs equal the sequential FP64 restatement of
+    SOM_Batch.train (last epoch's weights) to FP32 rounding."""
+    W0 = R.init_rand(4, 4, 4, seed=0)
+    X = golden_data(1000)
+    W, c1, qe = R.trajectory_fp64(W0, R.topography(4, 4, "square"), X, 2.0, 1)
+    assert W.dtype == np.float32
+    assert np.allclose(W.reshape(4, 4, 4), np.array(GOLD["centroids_epoch"]), rtol=GOLD["centroids_epoch_rtol"])
+    rng = np.random.default_rng(23)
+    Xh = rng.standard_normal((1500, 3)).astype(np.float32)
+    Wh0 = rng.standard_normal((20, 3)).astype(np.float32)
+    Wt, _, qet = R.trajectory_fp64(Wh0, R.topography(4, 5), Xh, 2.5, 3)
+    Ws, hist, qes, _ = R.train(Wh0, 4, 5, Xh, n_epochs=3, sigma=2.5, use_best_epoch=False, mode=FP64)
+    assert np.allclose(Wt, Ws, rtol=1e-6, atol=1e-6)
+    assert abs(qet - qes[-1]) <= 1e-5 * qes[-1]  # the QE of the final weights
