@@ -328,6 +328,15 @@ static_assert(NW_BF16 * G_BF16 * 32 == SOM_TILE_ROWS, "tile rows must match the 
 // D = 32 (more neurons closer than the window, on large smooth maps): 16 per lane plus a pool of 128
 // shared by the 32-sample group for lanes past that, in the same LDS; only a full pool sends a sample
 // to the fallback. The pool's spill test costs a few percent at D = 16, where lists rarely fill.
+using lds_u16 = unsigned short __attribute__((address_space(3)));
+// LDS byte address of an object in LDS
+template <class T>
+__device__ __forceinline__ unsigned int lds_addr(T *p)
+{
+    return (unsigned int)(size_t)(T __attribute__((address_space(3))) *)p;
+}
+constexpr unsigned short CAND_EMPTY = 0xFFFF;  // slots are < 65535 (checked at launch)
+
 template <int KC>
 struct Bf16Cand {
     static constexpr int CAPL = KC == 1 ? 20 : 16;  // own list per lane
@@ -548,11 +557,22 @@ __device__ __forceinline__ void bmu_bf16_body(const BmuModelDev &Md, const float
         m2[g] = __builtin_amdgcn_fmed3f(m1[g], m2[g], t);
         m1[g] = __builtin_amdgcn_fmed3f(m1[g], t, neg_inf);
     };
+    // Without a pool (D = 16), lp is the LDS byte address of this lane's next list entry, held at the
+    // spare last row once the list is full (a store then costs one add and one min); the spare row
+    // starts as CAND_EMPTY, which no slot takes, so a list that overflowed is told apart from one that
+    // filled exactly. With a pool (D = 32), cnt counts the lane's candidates (two fewer registers).
+    unsigned int lp[G], lcap[G];
     int cnt[G], pool_n[G];  // pool_n: wave-uniform fill of the group's shared pool
     bool lost[G];           // a candidate of this lane found both its list and the pool full (D = 32)
 #pragma unroll
     for (int g = 0; g < G; g++) {
-        cnt[g] = 0;
+        if constexpr (POOL == 0) {
+            lp[g] = lds_addr(&cw[g][0][lane]);
+            lcap[g] = lds_addr(&cw[g][CAPL][lane]);
+            cw[g][CAPL][lane] = CAND_EMPTY;
+        } else {
+            cnt[g] = 0;
+        }
         pool_n[g] = 0;
         lost[g] = false;
     }
@@ -589,8 +609,16 @@ __device__ __forceinline__ void bmu_bf16_body(const BmuModelDev &Md, const float
                             }
                         }
                         if (hit) {
-                            cw[g][min(cnt[g], CAPL)][lane] = slot;
-                            cnt[g] += 1;
+                            if constexpr (POOL == 0) {
+                                *(lds_u16 *)(size_t)lp[g] = slot;
+                                // orders the address update after the store, so it is made in place (no
+                                // move); the empty statement emits no instruction
+                                asm volatile("" : "+v"(lp[g]) : : "memory");
+                                lp[g] = min(lp[g] + (unsigned)sizeof(cw[g][0]), lcap[g]);
+                            } else {
+                                cw[g][min(cnt[g], CAPL)][lane] = slot;
+                                cnt[g] += 1;
+                            }
                         }
                     }
                 }
@@ -641,8 +669,15 @@ __device__ __forceinline__ void bmu_bf16_body(const BmuModelDev &Md, const float
         const bool valid_s = s < n;
         Top2 tp;
         tp.init();
-        const int mycnt = min(cnt[g], CAPL);
-        const bool ovf = POOL > 0 ? lost[g] : cnt[g] > CAPL;
+        int mycnt;
+        bool ovf;
+        if constexpr (POOL == 0) {
+            mycnt = (int)((lp[g] - lds_addr(&cw[g][0][lane])) / (unsigned)sizeof(cw[g][0]));
+            ovf = mycnt == CAPL && cw[g][CAPL][lane] != CAND_EMPTY;
+        } else {
+            mycnt = min(cnt[g], CAPL);
+            ovf = lost[g];
+        }
         if (valid_s && !ovf) {
             float xr[DP];
 #pragma unroll
@@ -1210,8 +1245,8 @@ int somhip_launch_bmu(somhip_model *M, const float *x, int64_t n, int precision,
         somhip_set_error("unknown precision");
         return SOMHIP_ERR_ARG;
     }
-    if (M->pblk * 32 > 65536) {
-        somhip_set_error("BF16_RERANK supports at most 65536 neurons");
+    if (M->pblk * 32 >= 65536) {
+        somhip_set_error("BF16_RERANK supports at most 65504 neuron slots (about 65,000 neurons)");
         return SOMHIP_ERR_UNSUPPORTED;
     }
     int64_t *fb_list = fb_list_ext;

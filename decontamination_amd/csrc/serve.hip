@@ -562,8 +562,8 @@ int somhip_server_add_model(somhip_server *S, somhip_model *M, int *model_id)
         somhip_set_error("add models before submitting requests");
         return SOMHIP_ERR_STATE;
     }
-    if (M->pblk * 32 > 65536) {
-        somhip_set_error("serving supports at most 65536 neurons");
+    if (M->pblk * 32 >= 65536) {
+        somhip_set_error("serving supports at most 65504 neuron slots (about 65,000 neurons)");
         return SOMHIP_ERR_UNSUPPORTED;
     }
     int rc = somhip_prepare(M);
