@@ -421,12 +421,11 @@ __device__ __forceinline__ void bmu_bf16_body(const BmuModelDev &Md, const float
 #pragma unroll
             for (int e = 0; e < 8; e++) {
                 const float v = L.xc[kc][e];
-                const unsigned short h = f32_to_bf16_bits(v);
-                const float vh = bf16_bits_to_f32(h);
-                const unsigned short l = f32_to_bf16_bits(v - vh);
-                // -2 scaling is exact in bf16 (exponent shift)
-                bh[g][kc][e] = __builtin_bit_cast(__bf16, f32_to_bf16_bits(-2.f * vh));
-                bl[g][kc][e] = __builtin_bit_cast(__bf16, f32_to_bf16_bits(-2.f * bf16_bits_to_f32(l)));
+                // round-to-nearest-even hi/lo split by the hardware conversion; the weight fragments
+                // carry the factor -2
+                const __bf16 h = (__bf16)v;
+                bh[g][kc][e] = h;
+                bl[g][kc][e] = (__bf16)(v - (float)h);
                 sx = fmaf(fabsf(v), wmax[kc * 16 + 8 * khalf + e], sx);
                 kk = fmaf(v, v, kk);
             }
@@ -972,8 +971,12 @@ __global__ __launch_bounds__(256) void prep_rows_kernel(const float *__restrict_
                         const float x = v[kc * 16 + half * 8 + e + s];
                         const unsigned short h = f32_to_bf16_bits(x);
                         const unsigned short l = f32_to_bf16_bits(__fsub_rn(x, bf16_bits_to_f32(h)));
-                        hp |= (unsigned int)h << (16 * s);
-                        lp |= (unsigned int)l << (16 * s);
+                        // the fragments hold -2 w'_hi and -2 w'_lo (exact in bf16: an exponent step), so
+                        // the MFMAs give -2 x'.w' from the sample's unscaled split
+                        const unsigned short h2 = f32_to_bf16_bits(-2.f * bf16_bits_to_f32(h));
+                        const unsigned short l2 = f32_to_bf16_bits(-2.f * bf16_bits_to_f32(l));
+                        hp |= (unsigned int)h2 << (16 * s);
+                        lp |= (unsigned int)l2 << (16 * s);
                     }
                     hw[e / 2] = hp;
                     lw[e / 2] = lp;
