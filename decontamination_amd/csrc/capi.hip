@@ -367,7 +367,8 @@ int somhip_model_create(somhip_ctx *C, int m, int n, int dim, int topology, cons
     somhip_model *M = new somhip_model();
     M->ctx = C;
     M->m = m; M->n = n; M->mn = m * n; M->dim = dim;
-    M->dpad = (dim + 15) / 16 * 16;
+    // padded dimension of the kernels' layouts: 16, 32, or 64 (the MFMA paths take K slices of 16)
+    M->dpad = dim <= 16 ? 16 : dim <= 32 ? 32 : 64;
     M->topology = topology;
     M->nblk = (M->mn + 31) / 32;
     // the reference's grid (som_abstract.py:81-111): neuron i*n + j at (i, j), or (i + 0.5 (j % 2),

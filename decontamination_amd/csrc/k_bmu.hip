@@ -761,8 +761,10 @@ __device__ __forceinline__ void bmu_bf16_body(const BmuModelDev &Md, const float
     }
 }
 
+// 4 waves per SIMD up to D = 32; at D = 64 the sample fragments and four K slices per block need more
+// registers, 2 waves per SIMD
 template <int DP, int G, int MODE, bool NEED_C2>
-__global__ __launch_bounds__(NW_BF16 * WAVE, 4) void bmu_bf16_kernel(BmuModelDev Md, const float *__restrict__ x,
+__global__ __launch_bounds__(NW_BF16 * WAVE, DP <= 32 ? 4 : 2) void bmu_bf16_kernel(BmuModelDev Md, const float *__restrict__ x,
                                                                  int64_t n, int default_index,
                                                                  int32_t *__restrict__ c1, int32_t *__restrict__ c2,
                                                                  double *__restrict__ d1,
@@ -1065,6 +1067,8 @@ void launch_exact_group(int mode, hipStream_t st, const BmuModelDev &Md, const f
 }
 
 constexpr int G_FP32 = 2;
+// 32-sample groups per wave of the FP32 kernel: one at D = 64, where two would spill
+constexpr int fp32_groups(int dpad) { return dpad <= 32 ? G_FP32 : 1; }
 
 __global__ void zero_u32_kernel(unsigned int *p) { *p = 0u; }
 
@@ -1072,9 +1076,10 @@ template <int DP>
 void launch_fp32(hipStream_t st, somhip_model *M, const float *x, int64_t n, int default_index, int32_t *c1,
                  int32_t *c2, double *d1, float *xmax_part)
 {
-    const int64_t per_block = (int64_t)BMU_WAVES * G_FP32 * 32;
+    constexpr int G = fp32_groups(DP);
+    const int64_t per_block = (int64_t)BMU_WAVES * G * 32;
     const int64_t blocks = (n + per_block - 1) / per_block;
-    hipLaunchKernelGGL((bmu_fp32_kernel<DP, G_FP32>), dim3((unsigned)blocks), dim3(BMU_WAVES * WAVE), 0, st, x, n,
+    hipLaunchKernelGGL((bmu_fp32_kernel<DP, G>), dim3((unsigned)blocks), dim3(BMU_WAVES * WAVE), 0, st, x, n,
                        M->dim, M->centre, (const f32x4 *)M->frag_f32, (const f32x4 *)M->norms_nat, M->nblk, M->mn,
                        default_index, c1, c2, d1, xmax_part);
 }
@@ -1165,8 +1170,8 @@ int somhip_prepare(somhip_model *M)
     if (M->prepared) return SOMHIP_OK;
     hipStream_t st = M->ctx->stream;
     TimerScope ts(M->ctx, TK_PREP);
-    if (M->dpad != 16 && M->dpad != 32) {
-        somhip_set_error("MFMA precisions support dim <= 32");
+    if (M->dpad != 16 && M->dpad != 32 && M->dpad != 64) {
+        somhip_set_error("MFMA precisions support dim <= 64");
         return SOMHIP_ERR_UNSUPPORTED;
     }
     hipLaunchKernelGGL(prep_centre_kernel, dim3(M->dpad), dim3(256), 0, st, M->w, M->mn, M->dim, M->dpad, M->centre,
@@ -1178,7 +1183,8 @@ int somhip_prepare(somhip_model *M)
                        (f32x4 *)M->wslot, M->norms_perm, M->norms_nat, (unsigned int *)M->wmax,               \
                        (unsigned int *)M->scal)
     if (M->dpad == 16) SOM_PREP(16);
-    else SOM_PREP(32);
+    else if (M->dpad == 32) SOM_PREP(32);
+    else SOM_PREP(64);
 #undef SOM_PREP
     SOM_HIP_TRY(hipGetLastError());
     M->prepared = true;
@@ -1206,7 +1212,7 @@ int64_t somhip_bmu_xmax_rows(int precision, int dpad, int64_t n)
     if (n <= 0) return 0;
     if (precision == SOMHIP_PREC_EXACT) return exact_blocks(n) * 4;
     if (precision == SOMHIP_PREC_FP32) {
-        const int64_t per_block = (int64_t)BMU_WAVES * G_FP32 * 32;
+        const int64_t per_block = (int64_t)BMU_WAVES * fp32_groups(dpad) * 32;
         return (n + per_block - 1) / per_block * BMU_WAVES;
     }
     return (n + SOM_TILE_ROWS - 1) / SOM_TILE_ROWS * NW_BF16;
@@ -1231,8 +1237,8 @@ int somhip_launch_bmu(somhip_model *M, const float *x, int64_t n, int precision,
         SOM_HIP_TRY(hipGetLastError());
         return SOMHIP_OK;
     }
-    if (dp > 32) {
-        somhip_set_error("MFMA precisions support dim <= 32; use precision EXACT");
+    if (dp > 64) {
+        somhip_set_error("MFMA precisions support dim <= 64");
         return SOMHIP_ERR_UNSUPPORTED;
     }
     int rc = somhip_prepare(M);
@@ -1240,7 +1246,8 @@ int somhip_launch_bmu(somhip_model *M, const float *x, int64_t n, int precision,
     if (precision == SOMHIP_PREC_FP32) {
         TimerScope ts(M->ctx, TK_BMU_FP32);
         if (dp == 16) launch_fp32<16>(st, M, x, n, default_index, c1, c2, d1, xmax_part);
-        else launch_fp32<32>(st, M, x, n, default_index, c1, c2, d1, xmax_part);
+        else if (dp == 32) launch_fp32<32>(st, M, x, n, default_index, c1, c2, d1, xmax_part);
+        else launch_fp32<64>(st, M, x, n, default_index, c1, c2, d1, xmax_part);
         SOM_HIP_TRY(hipGetLastError());
         return SOMHIP_OK;
     }
@@ -1265,7 +1272,8 @@ int somhip_launch_bmu(somhip_model *M, const float *x, int64_t n, int precision,
     {
         TimerScope ts(M->ctx, TK_BMU_BF16);
         if (dp == 16) launch_bf16<16>(arithmetic, need_c2, st, M, x, n, default_index, c1, c2, d1, fb_list, fb_count, xmax_part);
-        else launch_bf16<32>(arithmetic, need_c2, st, M, x, n, default_index, c1, c2, d1, fb_list, fb_count, xmax_part);
+        else if (dp == 32) launch_bf16<32>(arithmetic, need_c2, st, M, x, n, default_index, c1, c2, d1, fb_list, fb_count, xmax_part);
+        else launch_bf16<64>(arithmetic, need_c2, st, M, x, n, default_index, c1, c2, d1, fb_list, fb_count, xmax_part);
         SOM_HIP_TRY(hipGetLastError());
     }
     TimerScope tf(M->ctx, TK_BMU_FALLBACK);
@@ -1274,7 +1282,8 @@ int somhip_launch_bmu(somhip_model *M, const float *x, int64_t n, int precision,
     const int64_t fb_blocks = std::max<int64_t>(1, std::min<int64_t>(n, (int64_t)M->ctx->num_cus * 8));
     const BmuModelDev Md = somhip_model_dev(M);
     if (dp == 16) launch_exact_group<16>(arithmetic, st, Md, x, fb_list, fb_count, fb_blocks, default_index, c1, c2, d1);
-    else launch_exact_group<32>(arithmetic, st, Md, x, fb_list, fb_count, fb_blocks, default_index, c1, c2, d1);
+    else if (dp == 32) launch_exact_group<32>(arithmetic, st, Md, x, fb_list, fb_count, fb_blocks, default_index, c1, c2, d1);
+    else launch_exact_group<64>(arithmetic, st, Md, x, fb_list, fb_count, fb_blocks, default_index, c1, c2, d1);
     SOM_HIP_TRY(hipGetLastError());
     return SOMHIP_OK;
 }

@@ -114,7 +114,7 @@ int somhip_uploader_put(somhip_uploader *up, const void *x_host, size_t x_bytes,
 /* Model: weights (mn, dim) FP32 and topography (mn, 2) FP32 resident on the device.
  * Replaces SOM_Abstract._weights/_topography being copied to the device on every launch
  * (decontamination/jit/__init__.py:372-373). topography_host must be the reference's FP32
- * topography (som_abstract.py:103-111); dim <= 32 for the MFMA precisions, <= 64 for EXACT. */
+ * topography (som_abstract.py:103-111); dim <= 64 (padded to 16, 32 or 64). */
 int somhip_model_create(somhip_ctx *ctx, int m, int n, int dim, int topology, const float *topography_host,
                         somhip_model **out);
 int somhip_model_destroy(somhip_model *model);

@@ -93,6 +93,8 @@ CASES = [
     (45, 50, 32, 3000, "square", 12),
     (20, 30, 32, 3000, "square", 9),
     (7, 9, 40, 700, "square", 10),
+    (12, 10, 48, 2500, "hexagonal", 15),
+    (30, 30, 64, 3000, "square", 16),
     # coarse sections of several chunks: 3 at 80x80/D16; 10 four-block chunks at 100x100/D32, so
     # the BF16 search rescans 5 and records the other 5 in its main pass
     (80, 80, 16, 4000, "hexagonal", 13),
@@ -117,8 +119,6 @@ def make_case(m, n, dim, N, seed, ties=True, nans=True):
 @pytest.mark.parametrize("precision", PRECS)
 def test_winners_vs_oracle(case, precision):
     m, n, dim, N, topo, seed = case
-    if precision != "exact" and dim > 32:
-        pytest.skip("MFMA precisions support dim <= 32")
     W, X = make_case(m, n, dim, N, seed)
     som = dam.SOM_Batch(m, n, dim, topology=topo, precision=precision)
     som._weights[...] = W
@@ -137,8 +137,6 @@ def test_winners_vs_oracle(case, precision):
 @pytest.mark.parametrize("precision", EXACT_PRECS)
 def test_errors_vs_oracle(case, precision):
     m, n, dim, N, topo, seed = case
-    if precision != "exact" and dim > 32:
-        pytest.skip("MFMA precisions support dim <= 32")
     W, X = make_case(m, n, dim, N, seed)
     rho = np.random.default_rng(seed + 100).random(N)
     rho[::13] = 0.0
@@ -170,7 +168,7 @@ def test_activation_vs_oracle(case, precision):
     assert np.allclose(got, exp.astype(np.float32), rtol=1e-6, atol=0)
 
 
-@pytest.mark.parametrize("shape", [(40, 40, 16), (30, 35, 32)])
+@pytest.mark.parametrize("shape", [(40, 40, 16), (30, 35, 32), (20, 24, 64)])
 def test_bf16_window_overflow_falls_back_exactly(shape):
     """A map whose neurons are mostly exact duplicates puts hundreds of neurons inside every sample's
     window: the per-lane candidate lists overflow and every such sample goes through the exact
@@ -238,7 +236,8 @@ def test_one_step_vs_fp64_factorised(precision):
     assert rel.max() <= 1e-6, rel.max()
 
 
-@pytest.mark.parametrize("shape", [(64, 64, 16, "hexagonal"), (100, 100, 32, "hexagonal"), (90, 70, 20, "square")])
+@pytest.mark.parametrize("shape", [(64, 64, 16, "hexagonal"), (100, 100, 32, "hexagonal"), (90, 70, 20, "square"),
+                                   (50, 50, 64, "hexagonal")])
 def test_one_step_multi_slice_scatter_vs_fp64(shape):
     """Maps whose per-neuron table needs several LDS slices (the bucketed scatter): one weighted epoch
     with invalid rows and zero / negative / infinite weights, against the FP64 factorised oracle on the
@@ -266,7 +265,7 @@ def test_one_step_multi_slice_scatter_vs_fp64(shape):
     assert rel.max() <= 1e-6, rel.max()
 
 
-@pytest.mark.parametrize("shape", [(64, 64, 16, "hexagonal"), (32, 40, 32, "square")])
+@pytest.mark.parametrize("shape", [(64, 64, 16, "hexagonal"), (32, 40, 32, "square"), (24, 24, 48, "hexagonal")])
 def test_trained_map_bf16_equals_exact(shape):
     """On a map with trained structure (near-equal neighbour distances, the hard case for the BF16
     window): BF16 training equals exact training bit for bit, and the BF16 winners / errors on the
