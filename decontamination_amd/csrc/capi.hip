@@ -622,6 +622,9 @@ int somhip_batch_epoch_accumulate(somhip_accum *A, const float *x, const double 
     const int nparts = somhip_errors_nparts(n);
     Workspace &W = A->ws;
     const int64_t xrows = somhip_bmu_xmax_rows(precision, M->dpad, n);
+    // the previous call's BMUs of the same samples (the previous epoch) guide the BF16 search's grouping
+    const bool have_prev = precision == SOMHIP_PREC_BF16_RERANK && W.c1_x == (const void *)x && W.c1_n == n;
+    if (have_prev) std::swap(W.c1, W.c1b);
     if ((rc = W.c1.ensure(sizeof(int32_t) * n)) || (rc = W.c2.ensure(sizeof(int32_t) * n)) ||
         (rc = W.d1.ensure(sizeof(double) * n)) || (rc = W.part.ensure(sizeof(double) * 3 * nparts)) ||
         (rc = W.bounds.ensure(512)) || (rc = W.shifts.ensure(512)) ||
@@ -633,9 +636,13 @@ int somhip_batch_epoch_accumulate(somhip_accum *A, const float *x, const double 
     double *d1 = (double *)W.d1.p;
     const bool need_err = errors != nullptr;
     // the BMU search also writes per-wave max |x_k| rows, from which the fixed-point scales follow
+    W.c1_x = nullptr;  // c1 is being rewritten
     if ((rc = somhip_launch_bmu(M, x, n, precision, arithmetic, need_err ? 1 : 0, -1, c1, need_err ? c2 : nullptr,
-                                need_err ? d1 : nullptr, (float *)W.xmax.p)))
+                                need_err ? d1 : nullptr, (float *)W.xmax.p, nullptr, nullptr,
+                                have_prev ? (const int32_t *)W.c1b.p : nullptr)))
         return rc;
+    W.c1_x = x;
+    W.c1_n = n;
     if (need_err) {
         const double penalty = M->topology == SOMHIP_TOPOLOGY_SQUARE ? 2.0 : 1.0;
         TimerScope ts(C, TK_ERRORS);
@@ -715,6 +722,7 @@ int somhip_batch_iter_accumulate(somhip_accum *A, const float *x, const double *
         (rc = A->upd.ensure(sizeof(double) * (size_t)nsplit * M->mn * (M->dim + 1))))
         return rc;
     int32_t *c1 = (int32_t *)W.c1.p;
+    W.c1_x = nullptr;  // c1 no longer holds the epoch path's BMUs
     if ((rc = somhip_launch_bmu(M, x, n, precision, arithmetic, 0, -1, c1, nullptr, nullptr, nullptr))) return rc;
     TimerScope ts(C, TK_ITER);
     return somhip_launch_iter(C->stream, c1, x, M->dim, M->dpad, rho, n, cur_vector, n_vectors, sigma0, M->topo, M->mn,

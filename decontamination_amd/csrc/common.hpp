@@ -110,6 +110,7 @@ struct BmuModelDev {
     int nblk;               // 32-slot blocks of the permuted order
     int nblk_a;             // leading blocks that hold the coarse sub-lattice (== nblk: none)
     int mn, dim;
+    int ncol;               // map columns (neuron index = row * ncol + column)
 };
 
 // One unit of serving work: rows [row0, row0 + SOM_TILE_ROWS) of one request.
@@ -166,10 +167,17 @@ struct DevBuf {
 // Per-call device workspace (BMU outputs, reduction partials, fixed-point scales and tables).
 struct Workspace {
     DevBuf c1, c2, d1, part, bounds, shifts, table, xmax, maps, bucket;
+    // Training: c1b holds the BMUs of the previous epoch_accumulate call, whose samples were c1_x / c1_n;
+    // the BF16 search groups each tile's rows by them (a stale buffer only changes the grouping's speed).
+    DevBuf c1b;
+    const void *c1_x = nullptr;
+    int64_t c1_n = -1;
     void release()
     {
         c1.release(); c2.release(); d1.release(); part.release(); bounds.release(); shifts.release(); table.release();
-        xmax.release(); maps.release(); bucket.release();
+        xmax.release(); maps.release(); bucket.release(); c1b.release();
+        c1_x = nullptr;
+        c1_n = -1;
     }
 };
 
@@ -244,9 +252,12 @@ int somhip_prepare(somhip_model *mdl);
 // fixed-point scales of training; somhip_bmu_xmax_rows gives the number of rows written.
 // fb_list / fb_count (nullable, both or neither): the BF16 path's fallback workspace (n int64 entries and
 // one counter); when absent the context's grow-only workspace is used (and allocated if too small).
+// prev_c1 (nullable, must not alias c1): earlier BMUs of the same rows; the BF16 search groups each
+// tile's rows by them, which changes only its speed, never its results.
 int somhip_launch_bmu(somhip_model *mdl, const float *x, int64_t n, int precision, int arithmetic, int need_c2,
                       int default_index, int32_t *c1, int32_t *c2, double *d1, float *xmax_part,
-                      int64_t *fb_list = nullptr, unsigned int *fb_count = nullptr);
+                      int64_t *fb_list = nullptr, unsigned int *fb_count = nullptr,
+                      const int32_t *prev_c1 = nullptr);
 int64_t somhip_bmu_xmax_rows(int precision, int dpad, int64_t n);
 BmuModelDev somhip_model_dev(const somhip_model *M);
 // Serving form of the BF16 search: `grid` tiles from `tiles` (n == 0 tiles are skipped), then the exact

@@ -351,6 +351,7 @@ struct Bf16Smem {
     float norms[2][CB * 32];            // [buf][cb][row]
     unsigned short cand[NW_BF16][G_BF16][CAPL + 1][WAVE];  // last row: writes past a full list
     unsigned int pool[NW_BF16][G_BF16][Bf16Cand<KC>::POOL > 0 ? Bf16Cand<KC>::POOL : 1];  // (lane << 16) | slot
+    unsigned short rows[NW_BF16][G_BF16][32];  // tile row of each (wave, group, column), reloaded for the rerank
 };
 
 template <int DP, int G, int MODE, bool NEED_C2, bool GC>
@@ -358,7 +359,8 @@ __device__ __forceinline__ void bmu_bf16_body(const BmuModelDev &Md, const float
                                               int64_t tile_row0, int default_index, int32_t *__restrict__ c1_out,
                                               int32_t *__restrict__ c2_out, double *__restrict__ d1_out,
                                               int64_t *__restrict__ fb_list_out, unsigned int *__restrict__ fb_count,
-                                              int64_t fb_tag, Bf16Smem<DP / 16> &sm, float *__restrict__ xmax_part)
+                                              int64_t fb_tag, Bf16Smem<DP / 16> &sm, float *__restrict__ xmax_part,
+                                              const int32_t *__restrict__ prev_c1)
 {
     constexpr int KC = DP / 16;
     constexpr int CB = Bf16Smem<KC>::CB;
@@ -388,7 +390,60 @@ __device__ __forceinline__ void bmu_bf16_body(const BmuModelDev &Md, const float
     const int wave = tid / WAVE;
     const int col = lane & 31;
     const int khalf = lane >> 5;
-    const int64_t sbase = tile_row0 + (int64_t)wave * (G * 32);
+    // rl[g]: the tile row this lane's group g works on. Without earlier BMUs, slot order; with them, the
+    // tile's rows counting-sorted by the 8x8 map patch (in Morton order) of their earlier BMU, so a
+    // 32-sample group tends to share its nearest neurons and records candidates in fewer blocks. The
+    // scratch is the fragment buffer, which the first chunk load overwrites after a barrier.
+    constexpr int TR = NW_BF16 * G * 32;  // rows per tile
+    constexpr int NBK = 65;               // 64 patches + rows past the end
+    int rl[G];
+#pragma unroll
+    for (int g = 0; g < G; g++) rl[g] = wave * (G * 32) + g * 32 + col;
+    if (prev_c1) {
+        unsigned int *bcount = (unsigned int *)&sm.frag[0][0];
+        unsigned int *bcur = bcount + NBK;
+        unsigned short *ord = (unsigned short *)(bcur + NBK);
+        for (int b = tid; b < NBK; b += NT) bcount[b] = 0u;
+        __syncthreads();
+        const int mcol = max(Md.ncol, 1), mrow = max(Md.mn / mcol, 1);
+        int key[(TR + NT - 1) / NT];
+#pragma unroll
+        for (int u = 0; u < (TR + NT - 1) / NT; u++) {
+            const int r = tid + u * NT;
+            const int64_t row = tile_row0 + r;
+            int k = NBK - 1;
+            if (r < TR && row < n) {
+                const int c = min(max((int)aptr<GC>(prev_c1)[row], 0), Md.mn - 1);
+                const int a = c / mcol, b = c - a * mcol;
+                const int pa = a * 8 / mrow, pb = b * 8 / mcol;
+                k = 0;
+#pragma unroll
+                for (int bit = 0; bit < 3; bit++) k |= (((pa >> bit) & 1) << (2 * bit + 1)) | (((pb >> bit) & 1) << (2 * bit));
+            }
+            key[u] = k;
+            if (r < TR) atomicAdd(&bcount[k], 1u);
+        }
+        __syncthreads();
+        if (tid == 0) {
+            unsigned int acc = 0u;
+            for (int b = 0; b < NBK; b++) {
+                bcur[b] = acc;
+                acc += bcount[b];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < (TR + NT - 1) / NT; u++) {
+            const int r = tid + u * NT;
+            if (r < TR) ord[atomicAdd(&bcur[key[u]], 1u)] = (unsigned short)r;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int g = 0; g < G; g++) rl[g] = ord[rl[g]];
+    }
+    // kept in LDS rather than in registers across the block loop (both lanes of a column write the same)
+#pragma unroll
+    for (int g = 0; g < G; g++) sm.rows[wave][g][col] = (unsigned short)rl[g];
     unsigned short(*cw)[CAPL + 1][WAVE] = sm.cand[wave];
     const int nchunks = (nblk + CB - 1) / CB;
     const int64_t frag_units_total = (int64_t)nblk * KC * 2 * 64;
@@ -408,7 +463,7 @@ __device__ __forceinline__ void bmu_bf16_body(const BmuModelDev &Md, const float
         for (int e = 0; e < 8; e++) amax[kc][e] = 0.f;
 #pragma unroll
     for (int g = 0; g < G; g++) {
-        const int64_t s = sbase + g * 32 + col;
+        const int64_t s = tile_row0 + rl[g];
         SampleLoad<KC> L;
         load_sample<KC>(x, n, dim, s, lane, centre, L);
 #pragma unroll
@@ -664,7 +719,7 @@ __device__ __forceinline__ void bmu_bf16_body(const BmuModelDev &Md, const float
     // ---------------- exact rerank of the candidate set ----------------
 #pragma unroll
     for (int g = 0; g < G; g++) {
-        const int64_t s = sbase + g * 32 + col;
+        const int64_t s = tile_row0 + sm.rows[wave][g][col];
         const bool valid_s = s < n;
         Top2 tp;
         tp.init();
@@ -770,11 +825,12 @@ __global__ __launch_bounds__(NW_BF16 * WAVE, DP <= 32 ? 4 : 2) void bmu_bf16_ker
                                                                  double *__restrict__ d1,
                                                                  int64_t *__restrict__ fb_list,
                                                                  unsigned int *__restrict__ fb_count,
-                                                                 float *__restrict__ xmax_part)
+                                                                 float *__restrict__ xmax_part,
+                                                                 const int32_t *__restrict__ prev_c1)
 {
     __shared__ Bf16Smem<DP / 16> sm;
     bmu_bf16_body<DP, G, MODE, NEED_C2, false>(Md, x, n, (int64_t)blockIdx.x * SOM_TILE_ROWS, default_index, c1, c2,
-                                               d1, fb_list, fb_count, 0, sm, xmax_part);
+                                               d1, fb_list, fb_count, 0, sm, xmax_part, prev_c1);
 }
 
 // Serving form: one workgroup per tile; tiles of different requests and models share one launch.
@@ -790,7 +846,7 @@ __global__ __launch_bounds__(NW_BF16 * WAVE, 4) void bmu_bf16_tiles_kernel(const
     const BmuModelDev Md = models[T.model];
     // the descriptors' pointers are generic to the compiler: global_* accesses instead of flat ones
     bmu_bf16_body<DP, G, MODE, NEED_C2, true>(Md, T.x, T.n, T.row0, T.default_index, T.c1, T.c2, T.d1, fb_list,
-                                              fb_count, (int64_t)blockIdx.x << FB_TILE_SHIFT, sm, nullptr);
+                                              fb_count, (int64_t)blockIdx.x << FB_TILE_SHIFT, sm, nullptr, nullptr);
 }
 
 // Exact top-2 of one sample by one workgroup of FB_THREADS threads: thread t scans the permuted
@@ -1087,13 +1143,13 @@ void launch_fp32(hipStream_t st, somhip_model *M, const float *x, int64_t n, int
 template <int DP>
 void launch_bf16(int mode, bool need_c2, hipStream_t st, somhip_model *M, const float *x, int64_t n,
                  int default_index, int32_t *c1, int32_t *c2, double *d1, int64_t *fb_list, unsigned int *fb_count,
-                 float *xmax_part)
+                 float *xmax_part, const int32_t *prev_c1)
 {
     const int64_t blocks = (n + SOM_TILE_ROWS - 1) / SOM_TILE_ROWS;
     dim3 grid((unsigned)blocks), block(NW_BF16 * WAVE);
 #define SOM_BF(MO, C)                                                                                           \
     hipLaunchKernelGGL((bmu_bf16_kernel<DP, G_BF16, MO, C>), grid, block, 0, st, somhip_model_dev(M), x, n, default_index, c1, \
-                       c2, d1, fb_list, fb_count, xmax_part)
+                       c2, d1, fb_list, fb_count, xmax_part, prev_c1)
     if (mode == SOMHIP_RERANK_FP64) {
         if (need_c2) SOM_BF(SOMHIP_RERANK_FP64, true); else SOM_BF(SOMHIP_RERANK_FP64, false);
     } else {
@@ -1122,6 +1178,7 @@ BmuModelDev somhip_model_dev(const somhip_model *M)
     d.nblk_a = M->pblk_a;
     d.mn = M->mn;
     d.dim = M->dim;
+    d.ncol = M->n;
     return d;
 }
 
@@ -1220,7 +1277,7 @@ int64_t somhip_bmu_xmax_rows(int precision, int dpad, int64_t n)
 
 int somhip_launch_bmu(somhip_model *M, const float *x, int64_t n, int precision, int arithmetic, int need_c2,
                       int default_index, int32_t *c1, int32_t *c2, double *d1, float *xmax_part, int64_t *fb_list_ext,
-                      unsigned int *fb_count_ext)
+                      unsigned int *fb_count_ext, const int32_t *prev_c1)
 {
     if (n <= 0) return SOMHIP_OK;
     hipStream_t st = M->ctx->stream;
@@ -1271,9 +1328,9 @@ int somhip_launch_bmu(somhip_model *M, const float *x, int64_t n, int precision,
     }
     {
         TimerScope ts(M->ctx, TK_BMU_BF16);
-        if (dp == 16) launch_bf16<16>(arithmetic, need_c2, st, M, x, n, default_index, c1, c2, d1, fb_list, fb_count, xmax_part);
-        else if (dp == 32) launch_bf16<32>(arithmetic, need_c2, st, M, x, n, default_index, c1, c2, d1, fb_list, fb_count, xmax_part);
-        else launch_bf16<64>(arithmetic, need_c2, st, M, x, n, default_index, c1, c2, d1, fb_list, fb_count, xmax_part);
+        if (dp == 16) launch_bf16<16>(arithmetic, need_c2, st, M, x, n, default_index, c1, c2, d1, fb_list, fb_count, xmax_part, prev_c1);
+        else if (dp == 32) launch_bf16<32>(arithmetic, need_c2, st, M, x, n, default_index, c1, c2, d1, fb_list, fb_count, xmax_part, prev_c1);
+        else launch_bf16<64>(arithmetic, need_c2, st, M, x, n, default_index, c1, c2, d1, fb_list, fb_count, xmax_part, prev_c1);
         SOM_HIP_TRY(hipGetLastError());
     }
     TimerScope tf(M->ctx, TK_BMU_FALLBACK);
