@@ -445,6 +445,7 @@ int somhip_model_set_weights(somhip_model *M, const float *w, int on_device)
                                on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, M->ctx->stream));
     if (!on_device) SOM_HIP_TRY(hipStreamSynchronize(M->ctx->stream));
     M->prepared = false;
+    M->uploads++;  // earlier BMUs kept by accumulators no longer describe this map's schedule
     return SOMHIP_OK;
 }
 
@@ -623,7 +624,9 @@ int somhip_batch_epoch_accumulate(somhip_accum *A, const float *x, const double 
     Workspace &W = A->ws;
     const int64_t xrows = somhip_bmu_xmax_rows(precision, M->dpad, n);
     // the previous call's BMUs of the same samples (the previous epoch) guide the BF16 search's grouping
-    const bool have_prev = precision == SOMHIP_PREC_BF16_RERANK && W.c1_x == (const void *)x && W.c1_n == n;
+    // (not after an explicit weight upload: a new schedule starts without them, as a new train() does)
+    const bool have_prev = precision == SOMHIP_PREC_BF16_RERANK && W.c1_x == (const void *)x && W.c1_n == n &&
+                           W.c1_uploads == M->uploads;
     if (have_prev) std::swap(W.c1, W.c1b);
     if ((rc = W.c1.ensure(sizeof(int32_t) * n)) || (rc = W.c2.ensure(sizeof(int32_t) * n)) ||
         (rc = W.d1.ensure(sizeof(double) * n)) || (rc = W.part.ensure(sizeof(double) * 3 * nparts)) ||
@@ -643,6 +646,7 @@ int somhip_batch_epoch_accumulate(somhip_accum *A, const float *x, const double 
         return rc;
     W.c1_x = x;
     W.c1_n = n;
+    W.c1_uploads = M->uploads;
     if (need_err) {
         const double penalty = M->topology == SOMHIP_TOPOLOGY_SQUARE ? 2.0 : 1.0;
         TimerScope ts(C, TK_ERRORS);

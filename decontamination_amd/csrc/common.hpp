@@ -172,6 +172,7 @@ struct Workspace {
     DevBuf c1b;
     const void *c1_x = nullptr;
     int64_t c1_n = -1;
+    long long c1_uploads = -1;  // the model's upload count when c1 was written
     void release()
     {
         c1.release(); c2.release(); d1.release(); part.release(); bounds.release(); shifts.release(); table.release();
@@ -223,6 +224,7 @@ struct somhip_model {
     int pblk_a = 0;        // leading permuted blocks holding the coarse sub-lattice (== pblk: none)
     float *w = nullptr;    // (mn, dim) FP32 weights, row-major
     float *topo = nullptr; // (mn, 2) FP32 topography
+    long long uploads = 0; // weight uploads by the caller (set_weights); an epoch's own update does not count
     // prepared data (valid when prepared == true)
     bool prepared = false;
     float *centre = nullptr;        // (dpad) FP32 centroid of the weights
