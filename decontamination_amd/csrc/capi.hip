@@ -622,6 +622,7 @@ int somhip_batch_epoch_accumulate(somhip_accum *A, const float *x, const double 
     const int nwg = somhip_scatter_nwg(n, M->mn, nv, C->num_cus, SCATTER_TABLE_BUDGET);
     const int nparts = somhip_errors_nparts(n);
     Workspace &W = A->ws;
+    precision = somhip_bmu_precision(M, precision);  // the per-wave |x| rows depend on the kernel that runs
     const int64_t xrows = somhip_bmu_xmax_rows(precision, M->dpad, n);
     // the previous call's BMUs of the same samples (the previous epoch) guide the BF16 search's grouping
     // (not after an explicit weight upload: a new schedule starts without them, as a new train() does)

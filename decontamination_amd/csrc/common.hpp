@@ -250,6 +250,9 @@ int somhip_timer_collect(somhip_ctx *C);
 
 // Internal launchers (return SOMHIP_OK or an error code).
 int somhip_prepare(somhip_model *mdl);
+// The BMU precision a launch with `precision` runs on this model (BF16_RERANK maps beyond 65,504
+// neuron slots run the exact kernel, which returns the same winners).
+int somhip_bmu_precision(const somhip_model *mdl, int precision);
 // xmax_part (nullable): per-wave rows of dpad floats with max |x_k| over finite values, for the
 // fixed-point scales of training; somhip_bmu_xmax_rows gives the number of rows written.
 // fb_list / fb_count (nullable, both or neither): the BF16 path's fallback workspace (n int64 entries and

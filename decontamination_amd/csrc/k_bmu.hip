@@ -1275,11 +1275,21 @@ int64_t somhip_bmu_xmax_rows(int precision, int dpad, int64_t n)
     return (n + SOM_TILE_ROWS - 1) / SOM_TILE_ROWS * NW_BF16;
 }
 
+int somhip_bmu_precision(const somhip_model *M, int precision)
+{
+    // Candidate slots are 16-bit in LDS (0xFFFF marks an empty entry). BF16_RERANK promises the exact
+    // winners of its rerank arithmetic, which the exact kernel computes directly, so maps with more
+    // slots run that kernel.
+    if (precision == SOMHIP_PREC_BF16_RERANK && (int64_t)M->pblk * 32 >= 65536) return SOMHIP_PREC_EXACT;
+    return precision;
+}
+
 int somhip_launch_bmu(somhip_model *M, const float *x, int64_t n, int precision, int arithmetic, int need_c2,
                       int default_index, int32_t *c1, int32_t *c2, double *d1, float *xmax_part, int64_t *fb_list_ext,
                       unsigned int *fb_count_ext, const int32_t *prev_c1)
 {
     if (n <= 0) return SOMHIP_OK;
+    precision = somhip_bmu_precision(M, precision);
     hipStream_t st = M->ctx->stream;
     const int dp = M->dpad;
     if (precision == SOMHIP_PREC_EXACT) {
@@ -1311,10 +1321,6 @@ int somhip_launch_bmu(somhip_model *M, const float *x, int64_t n, int precision,
     if (precision != SOMHIP_PREC_BF16_RERANK) {
         somhip_set_error("unknown precision");
         return SOMHIP_ERR_ARG;
-    }
-    if (M->pblk * 32 >= 65536) {
-        somhip_set_error("BF16_RERANK supports at most 65504 neuron slots (about 65,000 neurons)");
-        return SOMHIP_ERR_UNSUPPORTED;
     }
     int64_t *fb_list = fb_list_ext;
     unsigned int *fb_count = fb_count_ext;

@@ -42,7 +42,8 @@ extern "C" {
 /* BMU search precision. */
 #define SOMHIP_PREC_EXACT 0       /* direct-difference scan on the vector ALUs, rerank arithmetic below */
 #define SOMHIP_PREC_FP32 1        /* plain FP32 MFMA distance GEMM |w|^2 - 2 x.w, argmin on its values */
-#define SOMHIP_PREC_BF16_RERANK 2 /* 3-term split-BF16 MFMA GEMM, provable candidate window, exact rerank */
+#define SOMHIP_PREC_BF16_RERANK 2 /* 3-term split-BF16 MFMA GEMM, provable candidate window, exact rerank;
+                                    maps beyond 65,504 neuron slots run PREC_EXACT (same winners) */
 
 /* Arithmetic of the exact distance (PREC_EXACT and the rerank of PREC_BF16_RERANK). */
 #define SOMHIP_RERANK_REF32 0 /* reference: FP32 difference and square, FP64 sum (algo/__init__.py:89-98) */

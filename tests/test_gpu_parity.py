@@ -187,6 +187,29 @@ def test_bf16_window_overflow_falls_back_exactly(shape):
         assert np.isclose(q, eq, rtol=1e-12, atol=0) and np.isclose(t, et, rtol=1e-12, atol=0)
 
 
+def test_bf16_rerank_beyond_16bit_slots():
+    """A 256x257 map (65,792 neurons) has more neuron slots than the BF16 kernel's 16-bit candidate
+    entries address; `bf16_rerank` then runs the exact kernel: the oracle's winners and errors, and
+    training equal to `exact` training bit for bit."""
+    m, n, dim, N = 256, 257, 8, 2000
+    print(f"bf16_rerank beyond 16-bit slots {m}x{n}/D{dim}", flush=True)
+    W, X = make_case(m, n, dim, N, 31)
+    som = dam.SOM_Batch(m, n, dim, precision="bf16_rerank")
+    som._weights[...] = W
+    assert np.array_equal(som.get_winners(X), R.find_bmus(W, X, REF32))
+    q, t = som.compute_errors(X)
+    eq, et = R.compute_errors(W, R.topography(m, n), X, None, "hexagonal", REF32)
+    assert np.isclose(q, eq, rtol=1e-12, atol=0) and np.isclose(t, et, rtol=1e-12, atol=0)
+    trained = []
+    for precision in ("exact", "bf16_rerank"):
+        s = dam.SOM_Batch(m, n, dim, precision=precision)
+        s._weights[...] = W
+        s.train(X, n_epochs=2, use_best_epoch=False)
+        trained.append(s)
+    assert np.array_equal(trained[0].weights, trained[1].weights)
+    assert np.array_equal(trained[0].quantization_errors, trained[1].quantization_errors)
+
+
 @pytest.mark.parametrize("precision", EXACT_PRECS)
 def test_fp64_arithmetic_matches_fp64_oracle(precision):
     W, X = make_case(12, 12, 16, 5000, 21)
