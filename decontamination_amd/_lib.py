@@ -46,6 +46,7 @@ I = ctypes.c_int
 I64 = ctypes.c_int64
 SZ = ctypes.c_size_t
 F = ctypes.c_float
+D = ctypes.c_double
 PP = ctypes.POINTER(ctypes.c_void_p)
 
 # name -> argtypes (every function returns int)
@@ -85,6 +86,16 @@ SIGNATURES = {
     "somhip_batch_epoch_update": [P, I64, I64, F, P],
     "somhip_batch_iter_accumulate": [P, P, P, I64, I64, I64, F, I, I],
     "somhip_batch_iter_update": [P, P],
+    "somhip_model_create_f64": [P, I, I, I, I, P, PP],
+    "somhip_model_set_weights_f64": [P, P, I],
+    "somhip_model_get_weights_f64": [P, P],
+    "somhip_find_bmus_f64": [P, P, I64, P],
+    "somhip_count_bmus_f64": [P, P, P, I64, P],
+    "somhip_compute_errors_f64": [P, P, P, I64, P],
+    "somhip_batch_epoch_accumulate_f64": [P, P, P, I64, P],
+    "somhip_batch_epoch_update_f64": [P, I64, I64, D, P],
+    "somhip_batch_iter_accumulate_f64": [P, P, P, I64, I64, I64, D],
+    "somhip_batch_iter_update_f64": [P, P],
     "somhip_host_alloc": [SZ, PP],
     "somhip_host_free": [P],
     "somhip_server_create": [P, SZ, I, PP],

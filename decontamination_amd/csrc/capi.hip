@@ -13,10 +13,16 @@ namespace {
 
 thread_local std::string g_last_error;
 
-inline int check_model(const somhip_model *M)
+// f64: the entry point is one of the float64 (_f64) forms
+inline int check_model(const somhip_model *M, bool f64 = false)
 {
     if (!M || !M->ctx) {
         somhip_set_error("null model");
+        return SOMHIP_ERR_ARG;
+    }
+    if (M->f64 != f64) {
+        somhip_set_error(M->f64 ? "float64 model: use the _f64 entry points"
+                                : "float32 model: the _f64 entry points take float64 models");
         return SOMHIP_ERR_ARG;
     }
     return SOMHIP_OK;
@@ -160,6 +166,13 @@ struct TimingOff {
     explicit TimingOff(somhip_ctx *c) : C(c), was(c->timer.enabled) { C->timer.enabled = false; }
     ~TimingOff() { C->timer.enabled = was; }
 };
+
+// sigma = sigma0 * asymptotic_decay(e, E) >= FLOAT_EPS (som_batch.py:107-110, algo/__init__.py:64-67)
+static double epoch_sigma(double sigma0, int64_t cur_epoch, int64_t n_epochs)
+{
+    const double sigma = sigma0 * (1.0 / (1.0 + 2.0 * (double)cur_epoch / (double)n_epochs));
+    return sigma < SOM_FLOAT_EPS ? SOM_FLOAT_EPS : sigma;
+}
 
 struct somhip_accum {
     somhip_model *model = nullptr;
@@ -340,7 +353,11 @@ int somhip_memset(somhip_ctx *C, void *dptr, int value, size_t bytes)
     return SOMHIP_OK;
 }
 
-int somhip_model_create(somhip_ctx *C, int m, int n, int dim, int topology, const float *topo_host, somhip_model **out)
+}  // extern "C"
+
+namespace {
+
+int check_model_shape(somhip_ctx *C, int m, int n, int dim, int topology, const void *topo_host, somhip_model **out)
 {
     if (!C || !out || !topo_host) {
         somhip_set_error("null argument");
@@ -363,6 +380,34 @@ int somhip_model_create(somhip_ctx *C, int m, int n, int dim, int topology, cons
         somhip_set_error("invalid topology");
         return SOMHIP_ERR_ARG;
     }
+    return SOMHIP_OK;
+}
+
+// The reference's grid (som_abstract.py:81-111): neuron i*n + j at (i, j), or (i + 0.5 (j % 2),
+// j sqrt(3)/2) rounded to the topography's type for hexagonal; only then does the neighbourhood
+// factor into rows and columns.
+template <class T>
+bool closed_form_grid(const T *topo, int m, int n, int topology)
+{
+    if (m > 2048 || n > 2048) return false;
+    for (int i = 0; i < m; i++)
+        for (int j = 0; j < n; j++) {
+            const T ex = topology == SOMHIP_TOPOLOGY_SQUARE ? (T)i : (T)i + (T)0.5 * (T)(j % 2);
+            const T ey = topology == SOMHIP_TOPOLOGY_SQUARE ? (T)j : (T)(j * 0.8660254037844386);
+            const T *t = topo + 2 * ((size_t)i * n + j);
+            if (!(t[0] == ex && t[1] == ey)) return false;
+        }
+    return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+int somhip_model_create(somhip_ctx *C, int m, int n, int dim, int topology, const float *topo_host, somhip_model **out)
+{
+    int rc0 = check_model_shape(C, m, n, dim, topology, topo_host, out);
+    if (rc0) return rc0;
     SOM_HIP_TRY(hipSetDevice(C->device));
     somhip_model *M = new somhip_model();
     M->ctx = C;
@@ -371,19 +416,7 @@ int somhip_model_create(somhip_ctx *C, int m, int n, int dim, int topology, cons
     M->dpad = dim <= 16 ? 16 : dim <= 32 ? 32 : 64;
     M->topology = topology;
     M->nblk = (M->mn + 31) / 32;
-    // the reference's grid (som_abstract.py:81-111): neuron i*n + j at (i, j), or (i + 0.5 (j % 2),
-    // j sqrt(3)/2) in FP32 for hexagonal; only then does the neighbourhood factor into rows and columns
-    {
-        bool sep = m <= 2048 && n <= 2048;
-        for (int i = 0; sep && i < m; i++)
-            for (int j = 0; sep && j < n; j++) {
-                const float ex = topology == SOMHIP_TOPOLOGY_SQUARE ? (float)i : (float)i + 0.5f * (float)(j % 2);
-                const float ey = topology == SOMHIP_TOPOLOGY_SQUARE ? (float)j : (float)(j * 0.8660254037844386);
-                const float *t = topo_host + 2 * ((size_t)i * n + j);
-                sep = t[0] == ex && t[1] == ey;
-            }
-        M->separable = sep;
-    }
+    M->separable = closed_form_grid(topo_host, m, n, topology);
     PermLayout layout = make_perm(m, n);
     M->pblk = layout.pblk;
     M->pblk_a = layout.pblk_a;
@@ -425,8 +458,8 @@ int somhip_model_destroy(somhip_model *M)
 {
     if (!M) return SOMHIP_OK;
     if (M->ctx) (void)hipStreamSynchronize(M->ctx->stream);
-    void *ptrs[] = {M->w,       M->topo,      M->centre,  M->wmax,     M->scal, M->norms_perm,
-                    M->norms_nat, M->frag_bf, M->frag_f32, M->wslot, M->perm};
+    void *ptrs[] = {M->w,         M->topo,    M->centre,   M->wmax,  M->scal, M->norms_perm,
+                    M->norms_nat, M->frag_bf, M->frag_f32, M->wslot, M->perm, M->w64, M->topo64};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     delete M;
@@ -526,7 +559,7 @@ int somhip_count_bmus(somhip_model *M, const float *x, const double *rho, int64_
     if ((rc = somhip_launch_bmu(M, x, n, precision, arithmetic, 0, 0, c1, nullptr, nullptr, nullptr))) return rc;
     {
         TimerScope ts(C, TK_ABSMAX);
-        if ((rc = somhip_launch_absmax(C->stream, nullptr, 0, 16, rho, 0, rho ? n : 0, (unsigned int *)W.bounds.p,
+        if ((rc = somhip_launch_absmax(C->stream, NO_ROWS, 0, 16, rho, 0, rho ? n : 0, (unsigned int *)W.bounds.p,
                                        C->num_cus)))
             return rc;
         if ((rc = somhip_launch_shifts(C->stream, (const unsigned int *)W.bounds.p, 0, 0, rho != nullptr, n,
@@ -534,7 +567,7 @@ int somhip_count_bmus(somhip_model *M, const float *x, const double *rho, int64_
             return rc;
     }
     TimerScope ts(C, TK_SCATTER);
-    return somhip_launch_scatter(C->stream, c1, nullptr, 0, rho, 0, n, (const int *)W.shifts.p, M->mn, 1, nwg,
+    return somhip_launch_scatter(C->stream, c1, NO_ROWS, 0, rho, 0, n, (const int *)W.shifts.p, M->mn, 1, nwg,
                                  (unsigned long long *)W.table.p, result, C->lds_per_block);
 }
 
@@ -565,7 +598,7 @@ int somhip_compute_errors(somhip_model *M, const float *x, const double *rho, in
 
 int somhip_accum_create(somhip_model *M, somhip_accum **out)
 {
-    int rc = check_model(M);
+    int rc = check_model(M, M && M->f64);
     if (rc) return rc;
     if (!out) return SOMHIP_ERR_ARG;
     somhip_accum *A = new somhip_accum();
@@ -605,8 +638,8 @@ int somhip_batch_epoch_accumulate(somhip_accum *A, const float *x, const double 
 {
     if (!A) return SOMHIP_ERR_ARG;
     somhip_model *M = A->model;
-    int rc = check_prec(precision, arithmetic);
-    if (rc) return rc;
+    int rc = check_model(M);
+    if (rc || (rc = check_prec(precision, arithmetic))) return rc;
     if (A->mode == 2) {
         somhip_set_error("accumulator holds vector-count sums; reset it first");
         return SOMHIP_ERR_STATE;
@@ -658,7 +691,7 @@ int somhip_batch_epoch_accumulate(somhip_accum *A, const float *x, const double 
     {
         TimerScope ts(C, TK_ABSMAX);
         if (rho &&
-            (rc = somhip_launch_absmax(C->stream, nullptr, 0, 16, rho, 1, n, (unsigned int *)W.bounds.p, C->num_cus)))
+            (rc = somhip_launch_absmax(C->stream, NO_ROWS, 0, 16, rho, 1, n, (unsigned int *)W.bounds.p, C->num_cus)))
             return rc;
         if ((rc = somhip_launch_shifts_parts(C->stream, (const float *)W.xmax.p, xrows, M->dpad, M->dim,
                                              rho ? (const unsigned int *)W.bounds.p : nullptr, n, (int *)W.shifts.p)))
@@ -681,10 +714,9 @@ int somhip_batch_epoch_update(somhip_accum *A, int64_t cur_epoch, int64_t n_epoc
         return SOMHIP_ERR_ARG;
     }
     somhip_model *M = A->model;
-    // sigma = sigma0 * asymptotic_decay(e, E) >= FLOAT_EPS (som_batch.py:107-110, algo/__init__.py:64-67)
-    double sigma = (double)sigma0 * (1.0 / (1.0 + 2.0 * (double)cur_epoch / (double)n_epochs));
-    if (sigma < SOM_FLOAT_EPS) sigma = SOM_FLOAT_EPS;
-    int rc;
+    int rc = check_model(M);
+    if (rc) return rc;
+    const double sigma = epoch_sigma(sigma0, cur_epoch, n_epochs);
     if (M->separable && somhip_update_separable_fits(M->m, M->n, M->dim)) {
         if ((rc = A->upd.ensure(sizeof(double) * 2 * (size_t)M->mn * (M->dim + 1)))) return rc;
         TimerScope ts(M->ctx, TK_UPDATE);
@@ -708,8 +740,8 @@ int somhip_batch_iter_accumulate(somhip_accum *A, const float *x, const double *
 {
     if (!A) return SOMHIP_ERR_ARG;
     somhip_model *M = A->model;
-    int rc = check_prec(precision, arithmetic);
-    if (rc) return rc;
+    int rc = check_model(M);
+    if (rc || (rc = check_prec(precision, arithmetic))) return rc;
     if (A->mode == 1) {
         somhip_set_error("accumulator holds epoch sums; reset it first");
         return SOMHIP_ERR_STATE;
@@ -742,9 +774,259 @@ int somhip_batch_iter_update(somhip_accum *A, float *history_out)
         return SOMHIP_ERR_STATE;
     }
     somhip_model *M = A->model;
-    int rc = somhip_launch_divide(M->ctx->stream, A->acc, M->mn, M->dim, M->w, history_out);
+    int rc = check_model(M);
+    if (rc) return rc;
+    rc = somhip_launch_divide(M->ctx->stream, A->acc, M->mn, M->dim, M->w, history_out);
     M->prepared = false;
     return rc;
+}
+
+// ------------------------------------------------------------------------------------------------
+// float64 SOMs (the reference's dtype=np.float64): data, weights and topography in FP64; the BMU
+// search is the exact FP64 scan, the sums are the same fixed-point scatter, the update writes FP64.
+// ------------------------------------------------------------------------------------------------
+int somhip_model_create_f64(somhip_ctx *C, int m, int n, int dim, int topology, const double *topo_host,
+                            somhip_model **out)
+{
+    int rc = check_model_shape(C, m, n, dim, topology, topo_host, out);
+    if (rc) return rc;
+    SOM_HIP_TRY(hipSetDevice(C->device));
+    somhip_model *M = new somhip_model();
+    M->ctx = C;
+    M->m = m; M->n = n; M->mn = m * n; M->dim = dim;
+    M->dpad = (dim + 15) / 16 * 16;
+    M->topology = topology;
+    M->f64 = true;
+    M->separable = closed_form_grid(topo_host, m, n, topology);
+    hipError_t e;
+    if ((e = hipMalloc((void **)&M->w64, sizeof(double) * M->mn * dim)) != hipSuccess ||
+        (e = hipMalloc((void **)&M->topo64, sizeof(double) * M->mn * 2)) != hipSuccess ||
+        (e = hipMemcpyAsync(M->topo64, topo_host, sizeof(double) * M->mn * 2, hipMemcpyHostToDevice, C->stream)) !=
+            hipSuccess ||
+        (e = hipMemsetAsync(M->w64, 0, sizeof(double) * M->mn * dim, C->stream)) != hipSuccess ||
+        (e = hipStreamSynchronize(C->stream)) != hipSuccess) {
+        somhip_set_error(std::string("float64 model: ") + hipGetErrorString(e));
+        somhip_model_destroy(M);
+        return SOMHIP_ERR_HIP;
+    }
+    *out = M;
+    return SOMHIP_OK;
+}
+
+int somhip_model_set_weights_f64(somhip_model *M, const double *w, int on_device)
+{
+    int rc = check_model(M, true);
+    if (rc) return rc;
+    if (!w) {
+        somhip_set_error("null weights");
+        return SOMHIP_ERR_ARG;
+    }
+    SOM_HIP_TRY(hipMemcpyAsync(M->w64, w, sizeof(double) * M->mn * M->dim,
+                               on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, M->ctx->stream));
+    if (!on_device) SOM_HIP_TRY(hipStreamSynchronize(M->ctx->stream));
+    M->uploads++;
+    return SOMHIP_OK;
+}
+
+int somhip_model_get_weights_f64(somhip_model *M, double *w_host)
+{
+    int rc = check_model(M, true);
+    if (rc) return rc;
+    if (!w_host) return SOMHIP_ERR_ARG;
+    SOM_HIP_TRY(hipMemcpyAsync(w_host, M->w64, sizeof(double) * M->mn * M->dim, hipMemcpyDeviceToHost,
+                               M->ctx->stream));
+    SOM_HIP_TRY(hipStreamSynchronize(M->ctx->stream));
+    return SOMHIP_OK;
+}
+
+int somhip_find_bmus_f64(somhip_model *M, const double *x, int64_t n, int32_t *out)
+{
+    int rc = check_model(M, true);
+    if (rc) return rc;
+    if (n < 0 || (n > 0 && (!x || !out))) {
+        somhip_set_error("invalid x/out/n");
+        return SOMHIP_ERR_ARG;
+    }
+    return somhip_launch_bmu_f64(M, x, n, 0, 0, out, nullptr, nullptr);
+}
+
+int somhip_count_bmus_f64(somhip_model *M, const double *x, const double *rho, int64_t n, double *result)
+{
+    int rc = check_model(M, true);
+    if (rc) return rc;
+    if (n < 0 || (n > 0 && (!x || !result))) {
+        somhip_set_error("invalid x/result/n");
+        return SOMHIP_ERR_ARG;
+    }
+    if (n == 0) return SOMHIP_OK;
+    somhip_ctx *C = M->ctx;
+    Workspace &W = C->ws;
+    const int nwg = somhip_scatter_nwg(n, M->mn, 1, C->num_cus, SCATTER_TABLE_BUDGET);
+    if ((rc = W.c1.ensure(sizeof(int32_t) * n)) || (rc = W.bounds.ensure(256)) || (rc = W.shifts.ensure(256)) ||
+        (rc = W.table.ensure((size_t)nwg * M->mn * 8)))
+        return rc;
+    int32_t *c1 = (int32_t *)W.c1.p;
+    if ((rc = somhip_launch_bmu_f64(M, x, n, 0, 0, c1, nullptr, nullptr))) return rc;
+    {
+        TimerScope ts(C, TK_ABSMAX);
+        if ((rc = somhip_launch_absmax(C->stream, NO_ROWS, 0, 16, rho, 0, rho ? n : 0,
+                                       (unsigned int *)W.bounds.p, C->num_cus)) ||
+            (rc = somhip_launch_shifts(C->stream, (const unsigned int *)W.bounds.p, 0, 0, rho != nullptr, n,
+                                       (int *)W.shifts.p)))
+            return rc;
+    }
+    TimerScope ts(C, TK_SCATTER);
+    return somhip_launch_scatter(C->stream, c1, NO_ROWS, 0, rho, 0, n, (const int *)W.shifts.p, M->mn,
+                                 1, nwg, (unsigned long long *)W.table.p, result, C->lds_per_block);
+}
+
+int somhip_compute_errors_f64(somhip_model *M, const double *x, const double *rho, int64_t n, double *errors)
+{
+    int rc = check_model(M, true);
+    if (rc) return rc;
+    if (n < 0 || (n > 0 && (!x || !errors))) {
+        somhip_set_error("invalid x/errors/n");
+        return SOMHIP_ERR_ARG;
+    }
+    if (n == 0) return SOMHIP_OK;
+    somhip_ctx *C = M->ctx;
+    Workspace &W = C->ws;
+    const int nparts = somhip_errors_nparts(n);
+    if ((rc = W.c1.ensure(sizeof(int32_t) * n)) || (rc = W.c2.ensure(sizeof(int32_t) * n)) ||
+        (rc = W.d1.ensure(sizeof(double) * n)) || (rc = W.part.ensure(sizeof(double) * 3 * nparts)))
+        return rc;
+    int32_t *c1 = (int32_t *)W.c1.p, *c2 = (int32_t *)W.c2.p;
+    double *d1 = (double *)W.d1.p;
+    if ((rc = somhip_launch_bmu_f64(M, x, n, 1, -1, c1, c2, d1))) return rc;
+    const double penalty = M->topology == SOMHIP_TOPOLOGY_SQUARE ? 2.0 : 1.0;
+    TimerScope ts(C, TK_ERRORS);
+    return somhip_launch_errors(C->stream, c1, c2, d1, rho, n, (const double *)M->topo64, penalty,
+                                (double *)W.part.p, nparts, errors);
+}
+
+int somhip_batch_epoch_accumulate_f64(somhip_accum *A, const double *x, const double *rho, int64_t n, double *errors)
+{
+    if (!A) return SOMHIP_ERR_ARG;
+    somhip_model *M = A->model;
+    int rc = check_model(M, true);
+    if (rc) return rc;
+    if (A->mode == 2) {
+        somhip_set_error("accumulator holds vector-count sums; reset it first");
+        return SOMHIP_ERR_STATE;
+    }
+    if (n < 0 || (n > 0 && !x)) {
+        somhip_set_error("invalid x/n");
+        return SOMHIP_ERR_ARG;
+    }
+    A->mode = 1;
+    if (n == 0) return SOMHIP_OK;
+    somhip_ctx *C = M->ctx;
+    const int nv = M->dim + 1;
+    const int nwg = somhip_scatter_nwg(n, M->mn, nv, C->num_cus, SCATTER_TABLE_BUDGET);
+    const int nparts = somhip_errors_nparts(n);
+    Workspace &W = A->ws;
+    if ((rc = W.c1.ensure(sizeof(int32_t) * n)) || (rc = W.c2.ensure(sizeof(int32_t) * n)) ||
+        (rc = W.d1.ensure(sizeof(double) * n)) || (rc = W.part.ensure(sizeof(double) * 3 * nparts)) ||
+        (rc = W.bounds.ensure(512)) || (rc = W.shifts.ensure(512)) ||
+        (rc = W.table.ensure((size_t)nwg * M->mn * nv * 8)) || (rc = W.bucket.ensure(sizeof(unsigned int) * (size_t)n)))
+        return rc;
+    int32_t *c1 = (int32_t *)W.c1.p, *c2 = (int32_t *)W.c2.p;
+    double *d1 = (double *)W.d1.p;
+    const bool need_err = errors != nullptr;
+    if ((rc = somhip_launch_bmu_f64(M, x, n, need_err ? 1 : 0, -1, c1, need_err ? c2 : nullptr,
+                                    need_err ? d1 : nullptr)))
+        return rc;
+    if (need_err) {
+        const double penalty = M->topology == SOMHIP_TOPOLOGY_SQUARE ? 2.0 : 1.0;
+        TimerScope ts(C, TK_ERRORS);
+        if ((rc = somhip_launch_errors(C->stream, c1, c2, d1, rho, n, (const double *)M->topo64, penalty,
+                                       (double *)W.part.p, nparts, errors)))
+            return rc;
+    }
+    {
+        // the density stays FP64 (the reference casts it to dtype); scales from max |x_k| and max rho
+        TimerScope ts(C, TK_ABSMAX);
+        if ((rc = somhip_launch_absmax(C->stream, x, M->dim, M->dpad, rho, 0, n, (unsigned int *)W.bounds.p,
+                                       C->num_cus)) ||
+            (rc = somhip_launch_shifts(C->stream, (const unsigned int *)W.bounds.p, M->dim, 1, rho != nullptr, n,
+                                       (int *)W.shifts.p)))
+            return rc;
+    }
+    TimerScope ts(C, TK_SCATTER);
+    return somhip_launch_scatter(C->stream, c1, x, M->dim, rho, 0, n, (const int *)W.shifts.p, M->mn, nv, nwg,
+                                 (unsigned long long *)W.table.p, A->acc, C->lds_per_block, (unsigned int *)W.bucket.p);
+}
+
+int somhip_batch_epoch_update_f64(somhip_accum *A, int64_t cur_epoch, int64_t n_epochs, double sigma0,
+                                  double *history_out)
+{
+    if (!A) return SOMHIP_ERR_ARG;
+    somhip_model *M = A->model;
+    int rc = check_model(M, true);
+    if (rc) return rc;
+    if (A->mode != 1) {
+        somhip_set_error("epoch update without epoch accumulation");
+        return SOMHIP_ERR_STATE;
+    }
+    if (n_epochs <= 0 || cur_epoch < 0) {
+        somhip_set_error("invalid epoch counters");
+        return SOMHIP_ERR_ARG;
+    }
+    const double sigma = epoch_sigma(sigma0, cur_epoch, n_epochs);
+    TimerScope ts(M->ctx, TK_UPDATE);
+    if (M->separable && somhip_update_separable_fits(M->m, M->n, M->dim)) {
+        if ((rc = A->upd.ensure(sizeof(double) * 2 * (size_t)M->mn * (M->dim + 1)))) return rc;
+        return somhip_launch_update_separable(M->ctx->stream, A->acc, (const double *)M->topo64, M->m, M->n,
+                                              M->topology == SOMHIP_TOPOLOGY_HEXAGONAL ? 1 : 0, M->dim, sigma,
+                                              (double *)A->upd.p, M->w64, history_out);
+    }
+    const int nsplit = somhip_update_nsplit(M->mn);
+    if ((rc = A->upd.ensure(sizeof(double) * (size_t)nsplit * M->mn * (M->dim + 1)))) return rc;
+    return somhip_launch_update(M->ctx->stream, A->acc, (const double *)M->topo64, M->mn, M->dim, sigma,
+                                (double *)A->upd.p, nsplit, M->w64, history_out);
+}
+
+int somhip_batch_iter_accumulate_f64(somhip_accum *A, const double *x, const double *rho, int64_t n,
+                                     int64_t cur_vector, int64_t n_vectors, double sigma0)
+{
+    if (!A) return SOMHIP_ERR_ARG;
+    somhip_model *M = A->model;
+    int rc = check_model(M, true);
+    if (rc) return rc;
+    if (A->mode == 1) {
+        somhip_set_error("accumulator holds epoch sums; reset it first");
+        return SOMHIP_ERR_STATE;
+    }
+    if (n < 0 || (n > 0 && !x) || n_vectors <= 0 || cur_vector < 0) {
+        somhip_set_error("invalid x/n/vector counters");
+        return SOMHIP_ERR_ARG;
+    }
+    A->mode = 2;
+    if (n == 0) return SOMHIP_OK;
+    somhip_ctx *C = M->ctx;
+    Workspace &W = A->ws;
+    const int nsplit = somhip_iter_nsplit(n, M->mn);
+    if ((rc = W.c1.ensure(sizeof(int32_t) * n)) ||
+        (rc = A->upd.ensure(sizeof(double) * (size_t)nsplit * M->mn * (M->dim + 1))))
+        return rc;
+    int32_t *c1 = (int32_t *)W.c1.p;
+    if ((rc = somhip_launch_bmu_f64(M, x, n, 0, -1, c1, nullptr, nullptr))) return rc;
+    TimerScope ts(C, TK_ITER);
+    return somhip_launch_iter(C->stream, c1, x, M->dim, M->dpad, rho, n, cur_vector, n_vectors, sigma0,
+                              (const double *)M->topo64, M->mn, (double *)A->upd.p, nsplit, A->acc);
+}
+
+int somhip_batch_iter_update_f64(somhip_accum *A, double *history_out)
+{
+    if (!A) return SOMHIP_ERR_ARG;
+    somhip_model *M = A->model;
+    int rc = check_model(M, true);
+    if (rc) return rc;
+    if (A->mode != 2) {
+        somhip_set_error("vector-count update without vector-count accumulation");
+        return SOMHIP_ERR_STATE;
+    }
+    return somhip_launch_divide(M->ctx->stream, A->acc, M->mn, M->dim, M->w64, history_out);
 }
 
 }  // extern "C"

@@ -64,6 +64,21 @@ __device__ __forceinline__ double sqd_fp64(const float *a, const float *b, int d
     return r;
 }
 
+// The same direct difference on FP64 operands (float64 SOMs: data, weights and topography in FP64).
+__device__ __forceinline__ double sqd_f64(const double *a, const double *b, int dim)
+{
+    double r = 0.0;
+    for (int k = 0; k < dim; k++) {
+        double d = __dsub_rn(a[k], b[k]);
+        r = __dadd_rn(r, __dmul_rn(d, d));
+    }
+    return r;
+}
+
+// square_distance_xpu of two grid positions in the topography's own type
+__device__ __forceinline__ double sqd_topo(const float *a, const float *b) { return sqd_ref32(a, b, 2); }
+__device__ __forceinline__ double sqd_topo(const double *a, const double *b) { return sqd_f64(a, b, 2); }
+
 template <int MODE>
 __device__ __forceinline__ double sqd_mode(const float *a, const float *b, int dim)
 {
@@ -224,6 +239,11 @@ struct somhip_model {
     int pblk_a = 0;        // leading permuted blocks holding the coarse sub-lattice (== pblk: none)
     float *w = nullptr;    // (mn, dim) FP32 weights, row-major
     float *topo = nullptr; // (mn, 2) FP32 topography
+    // float64 models (somhip_model_create_f64) hold their weights and topography here instead and have
+    // none of the FP32 buffers or prepared data
+    bool f64 = false;
+    double *w64 = nullptr;
+    double *topo64 = nullptr;
     long long uploads = 0; // weight uploads by the caller (set_weights); an epoch's own update does not count
     // prepared data (valid when prepared == true)
     bool prepared = false;
@@ -264,6 +284,9 @@ int somhip_launch_bmu(somhip_model *mdl, const float *x, int64_t n, int precisio
                       int64_t *fb_list = nullptr, unsigned int *fb_count = nullptr,
                       const int32_t *prev_c1 = nullptr);
 int64_t somhip_bmu_xmax_rows(int precision, int dpad, int64_t n);
+// Exact FP64 search of a float64 model (data and weights in FP64); same output contract.
+int somhip_launch_bmu_f64(somhip_model *mdl, const double *x, int64_t n, int need_c2, int default_index, int32_t *c1,
+                          int32_t *c2, double *d1);
 BmuModelDev somhip_model_dev(const somhip_model *M);
 // Serving form of the BF16 search: `grid` tiles from `tiles` (n == 0 tiles are skipped), then the exact
 // fallback over the flagged (tile, row) entries with a fixed grid; fb_count must be zero on entry.

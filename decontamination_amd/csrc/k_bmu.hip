@@ -51,18 +51,21 @@ __device__ __forceinline__ int xchg32i(int v) { return __shfl_xor(v, 32, WAVE); 
 __device__ __forceinline__ double xchg32d(double v) { return __shfl_xor(v, 32, WAVE); }
 
 // ------------------------------------------------------------------------------------------------
-// EXACT: direct-difference scan. Grid: one thread per listed sample.
+// EXACT: direct-difference scan. Grid: one thread per listed sample. T = float: the reference's
+// FP32-difference formula (MODE REF32) or FP64 differences of the FP32 values (MODE FP64); T = double
+// (float64 SOMs): FP64 differences of FP64 data and weights, no |x| maxima (xmax_part must be null).
 // ------------------------------------------------------------------------------------------------
-template <int DP, int MODE, bool NEED_C2>
-__global__ __launch_bounds__(256) void bmu_exact_kernel(const float *__restrict__ x, int64_t n, int dim,
+template <int DP, int MODE, bool NEED_C2, class T = float>
+__global__ __launch_bounds__(256) void bmu_exact_kernel(const T *__restrict__ x, int64_t n, int dim,
                                                         const int64_t *__restrict__ list,
                                                         const unsigned int *__restrict__ list_count,
-                                                        const float *__restrict__ w, int mn, int default_index,
+                                                        const T *__restrict__ w, int mn, int default_index,
                                                         int32_t *__restrict__ c1, int32_t *__restrict__ c2,
                                                         double *__restrict__ d1, float *__restrict__ xmax_part)
 {
-    constexpr int CH = 128;  // neurons per LDS chunk
-    __shared__ float ws[CH * DP];
+    constexpr bool F64 = sizeof(T) == sizeof(double);
+    constexpr int CH = F64 ? 64 : 128;  // neurons per LDS chunk
+    __shared__ T ws[CH * DP];
     const int64_t total = list ? (int64_t)(*list_count) : n;
     float amax[DP];
 #pragma unroll
@@ -72,12 +75,14 @@ __global__ __launch_bounds__(256) void bmu_exact_kernel(const float *__restrict_
         const int64_t t = base + threadIdx.x;
         const bool active = t < total;
         const int64_t i = active ? (list ? list[t] : t) : 0;
-        float xr[DP];
+        T xr[DP];
 #pragma unroll
         for (int k = 0; k < DP; k++) {
-            xr[k] = (active && k < dim) ? x[i * dim + k] : 0.f;
-            const float a = fabsf(xr[k]);
-            amax[k] = fmaxf(amax[k], a < __builtin_inff() ? a : 0.f);
+            xr[k] = (active && k < dim) ? x[i * dim + k] : (T)0;
+            if constexpr (!F64) {
+                const float a = fabsf(xr[k]);
+                amax[k] = fmaxf(amax[k], a < __builtin_inff() ? a : 0.f);
+            }
         }
         double md1 = 1.0e99, md2 = 1.0e99;
         int mi1 = -1, mi2 = -1;
@@ -85,12 +90,14 @@ __global__ __launch_bounds__(256) void bmu_exact_kernel(const float *__restrict_
             __syncthreads();
             for (int e = threadIdx.x; e < CH * DP; e += blockDim.x) {
                 int jj = j0 + e / DP, k = e % DP;
-                ws[e] = (jj < mn && k < dim) ? w[(int64_t)jj * dim + k] : 0.f;
+                ws[e] = (jj < mn && k < dim) ? w[(int64_t)jj * dim + k] : (T)0;
             }
             __syncthreads();
             const int jn = min(CH, mn - j0);
             for (int jj = 0; jj < jn; jj++) {
-                double d = sqd_mode<MODE>(&ws[jj * DP], xr, DP);  // zero padding adds exact zeros
+                double d;  // zero padding adds exact zeros
+                if constexpr (F64) d = sqd_f64(&ws[jj * DP], xr, DP);
+                else d = sqd_mode<MODE>(&ws[jj * DP], xr, DP);
                 if (d != d) continue;
                 const int j = j0 + jj;
                 if (NEED_C2) {
@@ -110,7 +117,7 @@ __global__ __launch_bounds__(256) void bmu_exact_kernel(const float *__restrict_
             if (d1) d1[i] = mi1 >= 0 ? md1 : __builtin_nan("");
         }
     }
-    if (xmax_part) {
+    if (!F64 && xmax_part) {
         const int lane = threadIdx.x & (WAVE - 1);
 #pragma unroll
         for (int k = 0; k < DP; k++) {
@@ -1110,6 +1117,19 @@ void launch_exact(int mode, bool need_c2, hipStream_t st, const float *x, int64_
 }
 
 template <int DP>
+void launch_exact64(bool need_c2, hipStream_t st, const double *x, int64_t n, int dim, const double *w, int mn,
+                    int default_index, int32_t *c1, int32_t *c2, double *d1)
+{
+    dim3 grid((unsigned)exact_blocks(n)), block(256);
+    if (need_c2)
+        hipLaunchKernelGGL((bmu_exact_kernel<DP, SOMHIP_RERANK_FP64, true, double>), grid, block, 0, st, x, n, dim,
+                           nullptr, nullptr, w, mn, default_index, c1, c2, d1, nullptr);
+    else
+        hipLaunchKernelGGL((bmu_exact_kernel<DP, SOMHIP_RERANK_FP64, false, double>), grid, block, 0, st, x, n, dim,
+                           nullptr, nullptr, w, mn, default_index, c1, c2, d1, nullptr);
+}
+
+template <int DP>
 void launch_exact_group(int mode, hipStream_t st, const BmuModelDev &Md, const float *x, const int64_t *list,
                         const unsigned int *list_count, int64_t blocks, int default_index, int32_t *c1, int32_t *c2,
                         double *d1)
@@ -1347,6 +1367,24 @@ int somhip_launch_bmu(somhip_model *M, const float *x, int64_t n, int precision,
     if (dp == 16) launch_exact_group<16>(arithmetic, st, Md, x, fb_list, fb_count, fb_blocks, default_index, c1, c2, d1);
     else if (dp == 32) launch_exact_group<32>(arithmetic, st, Md, x, fb_list, fb_count, fb_blocks, default_index, c1, c2, d1);
     else launch_exact_group<64>(arithmetic, st, Md, x, fb_list, fb_count, fb_blocks, default_index, c1, c2, d1);
+    SOM_HIP_TRY(hipGetLastError());
+    return SOMHIP_OK;
+}
+
+int somhip_launch_bmu_f64(somhip_model *M, const double *x, int64_t n, int need_c2, int default_index, int32_t *c1,
+                          int32_t *c2, double *d1)
+{
+    if (n <= 0) return SOMHIP_OK;
+    hipStream_t st = M->ctx->stream;
+    TimerScope ts(M->ctx, TK_BMU_EXACT);
+    const bool c2n = need_c2 != 0;
+    switch ((M->dim + 15) / 16 * 16) {
+        case 16: launch_exact64<16>(c2n, st, x, n, M->dim, M->w64, M->mn, default_index, c1, c2, d1); break;
+        case 32: launch_exact64<32>(c2n, st, x, n, M->dim, M->w64, M->mn, default_index, c1, c2, d1); break;
+        case 48: launch_exact64<48>(c2n, st, x, n, M->dim, M->w64, M->mn, default_index, c1, c2, d1); break;
+        case 64: launch_exact64<64>(c2n, st, x, n, M->dim, M->w64, M->mn, default_index, c1, c2, d1); break;
+        default: somhip_set_error("dim > 64 is not supported"); return SOMHIP_ERR_UNSUPPORTED;
+    }
     SOM_HIP_TRY(hipGetLastError());
     return SOMHIP_OK;
 }

@@ -18,23 +18,32 @@ constexpr int SCATTER_NT = 1024;                // threads per scatter workgroup
 
 // ------------------------------------------------------------------------------------------------
 // Column-wise max |v| over finite values; out[k] for k < dim is max|x_k|, out[dim] is max rho (as
-// float after the optional float rounding). Stored as float bits with atomicMax (values >= 0).
+// float after the optional float rounding). Stored as float bits with atomicMax (values >= 0); FP64
+// maxima are rounded up to a float, so the stored value stays a bound.
 // ------------------------------------------------------------------------------------------------
-template <int DP>
-__global__ __launch_bounds__(256) void absmax_kernel(const float *__restrict__ x, int dim, const double *__restrict__ rho,
+__device__ __forceinline__ float float_bound(float v) { return v; }
+__device__ __forceinline__ float float_bound(double v)
+{
+    float f = (float)v;
+    if ((double)f < v) f = nextafterf(f, __builtin_inff());
+    return f;
+}
+
+template <int DP, class T>
+__global__ __launch_bounds__(256) void absmax_kernel(const T *__restrict__ x, int dim, const double *__restrict__ rho,
                                                     int rho_f32, int64_t n, unsigned int *__restrict__ out)
 {
-    float mx[DP];
+    T mx[DP];
 #pragma unroll
-    for (int k = 0; k < DP; k++) mx[k] = 0.f;
+    for (int k = 0; k < DP; k++) mx[k] = (T)0;
     double mr = 0.0;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         if (x) {
 #pragma unroll
             for (int k = 0; k < DP; k++) {
                 if (k < dim) {
-                    float v = fabsf(x[i * dim + k]);
-                    if (v < __builtin_inff()) mx[k] = fmaxf(mx[k], v);
+                    const T v = fabs(x[i * dim + k]);
+                    if (v < (T)__builtin_inf()) mx[k] = fmax(mx[k], v);
                 }
             }
         }
@@ -50,7 +59,7 @@ __global__ __launch_bounds__(256) void absmax_kernel(const float *__restrict__ x
     if (x) {
 #pragma unroll
         for (int k = 0; k < DP; k++)
-            if (k < dim) atomicMax(&red[k], __builtin_bit_cast(unsigned int, mx[k]));
+            if (k < dim) atomicMax(&red[k], __builtin_bit_cast(unsigned int, float_bound(mx[k])));
     }
     // rho bound rounded up to a float so the scale stays safe
     float mrf = (float)mr;
@@ -140,6 +149,38 @@ __global__ __launch_bounds__(1024) void shifts_parts_kernel(const float *__restr
     }
 }
 
+// One sample row into registers, zero padded to DP; 16-byte vector loads when XVEC (dim a multiple
+// of the vector width, 16-byte aligned rows).
+template <int DP, bool XVEC>
+__device__ __forceinline__ void load_row(const float *xr, int dim, float (&xv)[DP])
+{
+    if (XVEC) {
+#pragma unroll
+        for (int v = 0; v < DP / 4; v++) {
+            const f32x4 t = 4 * v < dim ? ((const f32x4 *)xr)[v] : f32x4{0.f, 0.f, 0.f, 0.f};
+            xv[4 * v] = t[0]; xv[4 * v + 1] = t[1]; xv[4 * v + 2] = t[2]; xv[4 * v + 3] = t[3];
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < DP; k++) xv[k] = k < dim ? xr[k] : 0.f;
+    }
+}
+template <int DP, bool XVEC>
+__device__ __forceinline__ void load_row(const double *xr, int dim, double (&xv)[DP])
+{
+    typedef double f64x2 __attribute__((ext_vector_type(2)));
+    if (XVEC) {
+#pragma unroll
+        for (int v = 0; v < DP / 2; v++) {
+            const f64x2 t = 2 * v < dim ? ((const f64x2 *)xr)[v] : f64x2{0.0, 0.0};
+            xv[2 * v] = t[0]; xv[2 * v + 1] = t[1];
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < DP; k++) xv[k] = k < dim ? xr[k] : 0.0;
+    }
+}
+
 // ------------------------------------------------------------------------------------------------
 // Fixed-point scatter of per-sample rows into per-neuron integer sums.
 //   nv = dim + 1 columns (x != NULL: rho*x_0..rho*x_{dim-1}, rho) or nv = 1 (x == NULL: rho).
@@ -150,9 +191,9 @@ __global__ __launch_bounds__(1024) void shifts_parts_kernel(const float *__restr
 // BMU indices (coalesced), queues the ones inside the slice in a per-wave LDS ring, and whenever 64
 // are queued adds them with all lanes busy: one sample per lane, its row loaded as 16-byte vectors
 // when XVEC (dim % 4 == 0, 16-byte aligned rows), then one 64-bit LDS atomic per column.
-template <int DP, bool XVEC>
+template <int DP, bool XVEC, class T>
 __global__ __launch_bounds__(SCATTER_NT) void scatter_fixed_kernel(const int32_t *__restrict__ bmu,
-                                                                   const float *__restrict__ x, int dim,
+                                                                   const T *__restrict__ x, int dim,
                                                                    const double *__restrict__ rho, int rho_f32,
                                                                    int64_t n, int64_t per_wg,
                                                                    const int *__restrict__ shifts, int mn, int nv,
@@ -185,21 +226,12 @@ __global__ __launch_bounds__(SCATTER_NT) void scatter_fixed_kernel(const int32_t
             if (rho_f32) r = (double)(float)r;
             if (r > 0.0 && r < 1e300) {
                 if (x) {
-                    float xv[DP];
-                    if (XVEC) {
-                        const f32x4 *xr = (const f32x4 *)(x + i * dim);
-#pragma unroll
-                        for (int v = 0; v < DP / 4; v++) {
-                            const f32x4 t = 4 * v < dim ? xr[v] : f32x4{0.f, 0.f, 0.f, 0.f};
-                            xv[4 * v] = t[0]; xv[4 * v + 1] = t[1]; xv[4 * v + 2] = t[2]; xv[4 * v + 3] = t[3];
-                        }
-                    } else {
-#pragma unroll
-                        for (int k = 0; k < DP; k++) xv[k] = k < dim ? x[i * dim + k] : 0.f;
-                    }
+                    T xv[DP];
+                    load_row<DP, XVEC>(x + i * dim, dim, xv);
 #pragma unroll
                     for (int k = 0; k < DP; k++) {
-                        // r * x is exact in FP64 (24-bit mantissas), and so is the power-of-two scaling
+                        // r * x is exact in FP64 for FP32 data (24-bit mantissas; FP64 data rounds it
+                        // once), and so is the power-of-two scaling
                         if (k < dim)
                             atomicAdd(&row[k], (unsigned long long)llrint(ldexp(r * (double)xv[k], shifts[k])));
                     }
@@ -256,9 +288,9 @@ constexpr int BUCKET_MAX_SLICES = 1024;
 constexpr int BUCKET_LOCAL_BITS = 20;  // samples per workgroup range < 2^20
 constexpr int BUCKET_NEURON_BITS = 12; // neurons per slice <= 2^12
 
-template <int DP, bool XVEC>
+template <int DP, bool XVEC, class T>
 __global__ __launch_bounds__(SCATTER_NT) void scatter_bucket_kernel(const int32_t *__restrict__ bmu,
-                                                                    const float *__restrict__ x, int dim,
+                                                                    const T *__restrict__ x, int dim,
                                                                     const double *__restrict__ rho, int rho_f32,
                                                                     int64_t n, int64_t per_wg,
                                                                     const int *__restrict__ shifts, int mn, int nv,
@@ -314,21 +346,12 @@ __global__ __launch_bounds__(SCATTER_NT) void scatter_bucket_kernel(const int32_
             if (rho_f32) r = (double)(float)r;
             if (!(r > 0.0 && r < 1e300)) continue;
             if (x) {
-                float xv[DP];
-                if (XVEC) {
-                    const f32x4 *xr = (const f32x4 *)(x + i * dim);
-#pragma unroll
-                    for (int v = 0; v < DP / 4; v++) {
-                        const f32x4 t = 4 * v < dim ? xr[v] : f32x4{0.f, 0.f, 0.f, 0.f};
-                        xv[4 * v] = t[0]; xv[4 * v + 1] = t[1]; xv[4 * v + 2] = t[2]; xv[4 * v + 3] = t[3];
-                    }
-                } else {
-#pragma unroll
-                    for (int k = 0; k < DP; k++) xv[k] = k < dim ? x[i * dim + k] : 0.f;
-                }
+                T xv[DP];
+                load_row<DP, XVEC>(x + i * dim, dim, xv);
 #pragma unroll
                 for (int k = 0; k < DP; k++) {
-                    // r * x is exact in FP64 (24-bit mantissas), and so is the power-of-two scaling
+                    // r * x is exact in FP64 for FP32 data (24-bit mantissas; FP64 data rounds it once),
+                    // and so is the power-of-two scaling
                     if (k < dim) atomicAdd(&row[k], (unsigned long long)llrint(ldexp(r * (double)xv[k], shifts[k])));
                 }
             }
@@ -363,18 +386,18 @@ __global__ void scatter_reduce_kernel(const unsigned long long *__restrict__ par
 // ------------------------------------------------------------------------------------------------
 // Epoch-mode update: part[p][j][k] = sum_{c in split p} H_jc S_c[k], H_jc = exp(-|t_j-t_c|^2 inv2s2).
 // ------------------------------------------------------------------------------------------------
-template <int NV>
-__global__ __launch_bounds__(256) void update_partial_kernel(const double *__restrict__ S, const float *__restrict__ topo,
+template <int NV, class T>
+__global__ __launch_bounds__(256) void update_partial_kernel(const double *__restrict__ S, const T *__restrict__ topo,
                                                              int mn, int nv, double inv_2s2, int chunk,
                                                              double *__restrict__ part)
 {
     constexpr int TC = 64;
     __shared__ double s_S[TC * NV];
-    __shared__ float s_t[TC * 2];
+    __shared__ T s_t[TC * 2];
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     const int p = blockIdx.y;
     const int cbeg = p * chunk, cend = min(mn, cbeg + chunk);
-    float tj[2] = {0.f, 0.f};
+    T tj[2] = {(T)0, (T)0};
     if (j < mn) { tj[0] = topo[2 * j]; tj[1] = topo[2 * j + 1]; }
     double acc[NV];
 #pragma unroll
@@ -388,7 +411,7 @@ __global__ __launch_bounds__(256) void update_partial_kernel(const double *__res
         for (int cc = 0; cc < cl; cc++) {
             const double cnt = s_S[cc * NV + nv - 1];
             if (cnt == 0.0) continue;  // no sample has this BMU: S_c = 0, nothing to add
-            const double h = exp(-sqd_ref32(tj, &s_t[2 * cc], 2) * inv_2s2);
+            const double h = exp(-sqd_topo(tj, &s_t[2 * cc]) * inv_2s2);
 #pragma unroll
             for (int k = 0; k < NV; k++)
                 if (k < nv) acc[k] = __fma_rn(h, s_S[cc * NV + k], acc[k]);
@@ -436,9 +459,10 @@ __global__ __launch_bounds__(256) void update_sep_rows_kernel(const double *__re
     }
 }
 
-__global__ __launch_bounds__(256) void update_sep_cols_kernel(const double *__restrict__ T, const float *__restrict__ topo,
+template <class R>
+__global__ __launch_bounds__(256) void update_sep_cols_kernel(const double *__restrict__ T, const R *__restrict__ topo,
                                                               int m, int n, int dim, int hex, double inv_2s2,
-                                                              float *__restrict__ w, float *__restrict__ hist)
+                                                              R *__restrict__ w, R *__restrict__ hist)
 {
     extern __shared__ double sh[];
     const int nv = dim + 1;
@@ -447,10 +471,10 @@ __global__ __launch_bounds__(256) void update_sep_cols_kernel(const double *__re
     const int bj = blockIdx.x;
     const int np = hex ? 2 : 1;
     const int pj = hex ? (bj & 1) : 0;
-    const float yj = topo[2 * bj + 1];  // neuron (0, bj)
+    const R yj = topo[2 * bj + 1];  // neuron (0, bj)
     for (int t = threadIdx.x; t < n; t += blockDim.x) {
-        const float dy = __fsub_rn(yj, topo[2 * t + 1]);
-        gy[t] = exp(-(double)__fmul_rn(dy, dy) * inv_2s2);
+        const R tt[2] = {(R)0, yj}, tc[2] = {(R)0, topo[2 * t + 1]};
+        gy[t] = exp(-sqd_topo(tt, tc) * inv_2s2);  // fl(dy^2) in the topography's type
     }
     __syncthreads();
     for (int e = threadIdx.x; e < m * nv; e += blockDim.x) {
@@ -462,7 +486,7 @@ __global__ __launch_bounds__(256) void update_sep_cols_kernel(const double *__re
     for (int e = threadIdx.x; e < m * dim; e += blockDim.x) {
         const int aj = e / dim, k = e % dim;
         const double den = num[aj * nv + dim];
-        const float v = den > SOM_FLOAT_EPS ? (float)(num[aj * nv + k] / den) : 0.f;
+        const R v = den > SOM_FLOAT_EPS ? (R)(num[aj * nv + k] / den) : (R)0;
         const int64_t o = ((int64_t)aj * n + bj) * dim + k;
         w[o] = v;
         if (hist) hist[o] = v;
@@ -471,8 +495,9 @@ __global__ __launch_bounds__(256) void update_sep_cols_kernel(const double *__re
 
 // w_j = num_j / den_j if den_j > 1e-12 else 0 (som_batch.py:354-359); num/den = fixed-order sums of
 // the split partials (or the accumulator itself when nsplit == 1 and part == acc).
-__global__ void divide_kernel(const double *__restrict__ part, int nsplit, int mn, int dim, float *__restrict__ w,
-                              float *__restrict__ hist)
+template <class R>
+__global__ void divide_kernel(const double *__restrict__ part, int nsplit, int mn, int dim, R *__restrict__ w,
+                              R *__restrict__ hist)
 {
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= (int64_t)mn * dim) return;
@@ -483,7 +508,7 @@ __global__ void divide_kernel(const double *__restrict__ part, int nsplit, int m
         num += part[((int64_t)p * mn + j) * nv + k];
         den += part[((int64_t)p * mn + j) * nv + dim];
     }
-    const float v = den > SOM_FLOAT_EPS ? (float)(num / den) : 0.f;
+    const R v = den > SOM_FLOAT_EPS ? (R)(num / den) : (R)0;
     w[e] = v;
     if (hist) hist[e] = v;
 }
@@ -492,23 +517,23 @@ __global__ void divide_kernel(const double *__restrict__ part, int nsplit, int m
 // Vector-count mode: part[p][j][k] = sum_{i in split p} rho_i exp(-|t_j - t_bmu_i|^2 / (2 sigma_i^2)) x_ik
 // with sigma_i = sigma0 / (1 + 2 (cur + i) / n_vectors) >= 1e-12 (som_batch.py:154-210).
 // ------------------------------------------------------------------------------------------------
-template <int DP>
-__global__ __launch_bounds__(256) void iter_partial_kernel(const int32_t *__restrict__ bmu, const float *__restrict__ x,
+template <int DP, class T>
+__global__ __launch_bounds__(256) void iter_partial_kernel(const int32_t *__restrict__ bmu, const T *__restrict__ x,
                                                            int dim, const double *__restrict__ rho, int64_t n,
-                                                           int64_t cur, int64_t nvec, float sigma0,
-                                                           const float *__restrict__ topo, int mn, int64_t chunk,
+                                                           int64_t cur, int64_t nvec, T sigma0,
+                                                           const T *__restrict__ topo, int mn, int64_t chunk,
                                                            double *__restrict__ part)
 {
     constexpr int TS = 64;
-    __shared__ float s_x[TS * DP];
-    __shared__ float s_tb[TS * 2];
-    __shared__ double s_h0[TS];   // rho_i (float-rounded, as the reference casts density to dtype)
+    __shared__ T s_x[TS * DP];
+    __shared__ T s_tb[TS * 2];
+    __shared__ double s_h0[TS];   // rho_i (rounded to the data type, as the reference casts density to dtype)
     __shared__ double s_inv[TS];  // 1 / (2 sigma_i^2)
     __shared__ int s_ok[TS];
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     const int p = blockIdx.y;
     const int64_t ibeg = (int64_t)p * chunk, iend = min(n, ibeg + chunk);
-    float tj[2] = {0.f, 0.f};
+    T tj[2] = {(T)0, (T)0};
     if (j < mn) { tj[0] = topo[2 * j]; tj[1] = topo[2 * j + 1]; }
     double acc[DP + 1];
 #pragma unroll
@@ -518,16 +543,16 @@ __global__ __launch_bounds__(256) void iter_partial_kernel(const int32_t *__rest
         __syncthreads();
         for (int e = threadIdx.x; e < il * DP; e += blockDim.x) {
             const int ii = e / DP, k = e % DP;
-            s_x[e] = k < dim ? x[(i0 + ii) * dim + k] : 0.f;
+            s_x[e] = k < dim ? x[(i0 + ii) * dim + k] : (T)0;
         }
         for (int ii = threadIdx.x; ii < il; ii += blockDim.x) {
             const int64_t i = i0 + ii;
             const int c = bmu[i];
-            const float r = rho ? (float)rho[i] : 1.0f;
-            const bool ok = c >= 0 && r > 0.0f;
+            const T r = rho ? (T)rho[i] : (T)1;
+            const bool ok = c >= 0 && r > (T)0;
             s_ok[ii] = ok;
-            s_tb[2 * ii] = ok ? topo[2 * c] : 0.f;
-            s_tb[2 * ii + 1] = ok ? topo[2 * c + 1] : 0.f;
+            s_tb[2 * ii] = ok ? topo[2 * c] : (T)0;
+            s_tb[2 * ii + 1] = ok ? topo[2 * c + 1] : (T)0;
             double sg = (double)sigma0 * (1.0 / (1.0 + 2.0 * (double)(cur + i) / (double)nvec));
             if (sg < SOM_FLOAT_EPS) sg = SOM_FLOAT_EPS;
             s_inv[ii] = 1.0 / (2.0 * sg * sg);
@@ -536,7 +561,7 @@ __global__ __launch_bounds__(256) void iter_partial_kernel(const int32_t *__rest
         __syncthreads();
         for (int ii = 0; ii < il; ii++) {
             if (!s_ok[ii]) continue;
-            const double h = s_h0[ii] * exp(-sqd_ref32(tj, &s_tb[2 * ii], 2) * s_inv[ii]);
+            const double h = s_h0[ii] * exp(-sqd_topo(tj, &s_tb[2 * ii]) * s_inv[ii]);
 #pragma unroll
             for (int k = 0; k < DP; k++)
                 if (k < dim) acc[k] = __fma_rn(h, (double)s_x[ii * DP + k], acc[k]);
@@ -565,9 +590,10 @@ __global__ void sum_parts_kernel(const double *__restrict__ part, int nsplit, in
 // ------------------------------------------------------------------------------------------------
 // Errors (som_abstract.py:815-878): per-workgroup fixed-order sums, then one fixed-order final sum.
 // ------------------------------------------------------------------------------------------------
+template <class T>
 __global__ __launch_bounds__(256) void errors_partial_kernel(const int32_t *__restrict__ c1, const int32_t *__restrict__ c2,
                                                              const double *__restrict__ d1, const double *__restrict__ rho,
-                                                             int64_t n, const float *__restrict__ topo, double penalty,
+                                                             int64_t n, const T *__restrict__ topo, double penalty,
                                                              int64_t per_wg, double *__restrict__ part)
 {
     __shared__ double red[4][3];
@@ -579,7 +605,7 @@ __global__ __launch_bounds__(256) void errors_partial_kernel(const int32_t *__re
         const int a = c1[i], b = c2[i];
         if (a < 0 || b < 0 || a == b) continue;
         q += r * sqrt(d1[i]);
-        if (sqd_ref32(&topo[2 * a], &topo[2 * b], 2) > penalty) t += r;
+        if (sqd_topo(&topo[2 * a], &topo[2 * b]) > penalty) t += r;
         s += r;
     }
     // fixed-order reduction: butterfly within each wave, then the 4 waves in order
@@ -618,7 +644,8 @@ __global__ __launch_bounds__(256) void errors_final_kernel(const double *__restr
 // ------------------------------------------------------------------------------------------------
 // Host launchers used by capi.hip.
 // ------------------------------------------------------------------------------------------------
-int somhip_launch_absmax(hipStream_t st, const float *x, int dim, int dpad, const double *rho, int rho_f32, int64_t n,
+template <class T>
+int somhip_launch_absmax(hipStream_t st, const T *x, int dim, int dpad, const double *rho, int rho_f32, int64_t n,
                          unsigned int *bounds, int num_cus)
 {
     SOM_HIP_TRY(hipMemsetAsync(bounds, 0, sizeof(unsigned int) * (dim + 1), st));
@@ -626,10 +653,10 @@ int somhip_launch_absmax(hipStream_t st, const float *x, int dim, int dpad, cons
     const int64_t blocks = std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, (int64_t)num_cus * 8));
     dim3 g((unsigned)blocks), b(256);
     switch (dpad) {
-        case 16: hipLaunchKernelGGL(absmax_kernel<16>, g, b, 0, st, x, dim, rho, rho_f32, n, bounds); break;
-        case 32: hipLaunchKernelGGL(absmax_kernel<32>, g, b, 0, st, x, dim, rho, rho_f32, n, bounds); break;
-        case 48: hipLaunchKernelGGL(absmax_kernel<48>, g, b, 0, st, x, dim, rho, rho_f32, n, bounds); break;
-        default: hipLaunchKernelGGL(absmax_kernel<64>, g, b, 0, st, x, dim, rho, rho_f32, n, bounds); break;
+        case 16: hipLaunchKernelGGL((absmax_kernel<16, T>), g, b, 0, st, x, dim, rho, rho_f32, n, bounds); break;
+        case 32: hipLaunchKernelGGL((absmax_kernel<32, T>), g, b, 0, st, x, dim, rho, rho_f32, n, bounds); break;
+        case 48: hipLaunchKernelGGL((absmax_kernel<48, T>), g, b, 0, st, x, dim, rho, rho_f32, n, bounds); break;
+        default: hipLaunchKernelGGL((absmax_kernel<64, T>), g, b, 0, st, x, dim, rho, rho_f32, n, bounds); break;
     }
     SOM_HIP_TRY(hipGetLastError());
     return SOMHIP_OK;
@@ -667,7 +694,8 @@ int somhip_scatter_nwg(int64_t n, int mn, int nv, int num_cus, size_t table_budg
     return (int)std::max<int64_t>(1, w);
 }
 
-int somhip_launch_scatter(hipStream_t st, const int32_t *bmu, const float *x, int dim, const double *rho, int rho_f32,
+template <class T>
+int somhip_launch_scatter(hipStream_t st, const int32_t *bmu, const T *x, int dim, const double *rho, int rho_f32,
                           int64_t n, const int *shifts, int mn, int nv, int nwg, unsigned long long *partial,
                           double *acc, size_t lds_limit, unsigned int *bucket)
 {
@@ -687,7 +715,8 @@ int somhip_launch_scatter(hipStream_t st, const int32_t *bmu, const float *x, in
         return SOMHIP_ERR_UNSUPPORTED;
     }
     const int dpad = (dim + 15) / 16 * 16;
-    const bool xvec = x && dim % 4 == 0 && ((uintptr_t)x & 15) == 0;
+    constexpr int vec = 16 / (int)sizeof(T);  // values per 16-byte load
+    const bool xvec = x && dim % vec == 0 && ((uintptr_t)x & 15) == 0;
     // several slices: the bucketed kernel, with a power-of-two slice that fits beside its static arrays
     int log2_slice = 0, nslices = 0;
     if (bucket && several_slices && per_wg < (1ll << BUCKET_LOCAL_BITS)) {
@@ -700,7 +729,7 @@ int somhip_launch_scatter(hipStream_t st, const int32_t *bmu, const float *x, in
         const size_t lds = ((size_t)1 << log2_slice) * nv * 8;
 #define SOM_SB(D)                                                                                              \
     do {                                                                                                       \
-        auto kern = xvec ? scatter_bucket_kernel<D, true> : scatter_bucket_kernel<D, false>;                   \
+        auto kern = xvec ? scatter_bucket_kernel<D, true, T> : scatter_bucket_kernel<D, false, T>;             \
         SOM_HIP_TRY(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds)); \
         hipLaunchKernelGGL(kern, dim3(nwg), dim3(SCATTER_NT), lds, st, bmu, x, dim, rho, rho_f32, n, per_wg,     \
                            shifts, mn, nv, log2_slice, nslices, bucket, partial);                               \
@@ -714,7 +743,7 @@ int somhip_launch_scatter(hipStream_t st, const int32_t *bmu, const float *x, in
         const size_t lds = (size_t)slice_len * nv * 8 + queue_bytes;
 #define SOM_SC(D)                                                                                              \
     do {                                                                                                       \
-        auto kern = xvec ? scatter_fixed_kernel<D, true> : scatter_fixed_kernel<D, false>;                     \
+        auto kern = xvec ? scatter_fixed_kernel<D, true, T> : scatter_fixed_kernel<D, false, T>;               \
         SOM_HIP_TRY(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds)); \
         hipLaunchKernelGGL(kern, dim3(nwg), dim3(SCATTER_NT), lds, st, bmu, x, dim, rho, rho_f32, n, per_wg,     \
                            shifts, mn, nv, slice_len, partial);                                                \
@@ -740,20 +769,21 @@ int somhip_update_nsplit(int mn)
     return std::min(s, std::max(1, mn / 16));
 }
 
-int somhip_launch_update(hipStream_t st, const double *S, const float *topo, int mn, int dim, double sigma,
-                         double *part, int nsplit, float *w, float *hist)
+template <class T>
+int somhip_launch_update(hipStream_t st, const double *S, const T *topo, int mn, int dim, double sigma,
+                         double *part, int nsplit, T *w, T *hist)
 {
     const int nv = dim + 1;
     const double inv_2s2 = 1.0 / (2.0 * sigma * sigma);
     const int chunk = (mn + nsplit - 1) / nsplit;
     dim3 g((mn + 255) / 256, nsplit), b(256);
-    if (nv <= 17) hipLaunchKernelGGL(update_partial_kernel<17>, g, b, 0, st, S, topo, mn, nv, inv_2s2, chunk, part);
-    else if (nv <= 33) hipLaunchKernelGGL(update_partial_kernel<33>, g, b, 0, st, S, topo, mn, nv, inv_2s2, chunk, part);
-    else if (nv <= 49) hipLaunchKernelGGL(update_partial_kernel<49>, g, b, 0, st, S, topo, mn, nv, inv_2s2, chunk, part);
-    else hipLaunchKernelGGL(update_partial_kernel<65>, g, b, 0, st, S, topo, mn, nv, inv_2s2, chunk, part);
+    if (nv <= 17) hipLaunchKernelGGL((update_partial_kernel<17, T>), g, b, 0, st, S, topo, mn, nv, inv_2s2, chunk, part);
+    else if (nv <= 33) hipLaunchKernelGGL((update_partial_kernel<33, T>), g, b, 0, st, S, topo, mn, nv, inv_2s2, chunk, part);
+    else if (nv <= 49) hipLaunchKernelGGL((update_partial_kernel<49, T>), g, b, 0, st, S, topo, mn, nv, inv_2s2, chunk, part);
+    else hipLaunchKernelGGL((update_partial_kernel<65, T>), g, b, 0, st, S, topo, mn, nv, inv_2s2, chunk, part);
     SOM_HIP_TRY(hipGetLastError());
     const int64_t tot = (int64_t)mn * dim;
-    hipLaunchKernelGGL(divide_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, part, nsplit, mn, dim, w,
+    hipLaunchKernelGGL(divide_kernel<T>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, part, nsplit, mn, dim, w,
                        hist);
     SOM_HIP_TRY(hipGetLastError());
     return SOMHIP_OK;
@@ -766,8 +796,9 @@ bool somhip_update_separable_fits(int m, int n, int dim)
            sizeof(double) * ((size_t)n + (size_t)m * nv) <= 65536;
 }
 
-int somhip_launch_update_separable(hipStream_t st, const double *S, const float *topo, int m, int n, int hex, int dim,
-                                   double sigma, double *T, float *w, float *hist)
+template <class R>
+int somhip_launch_update_separable(hipStream_t st, const double *S, const R *topo, int m, int n, int hex, int dim,
+                                   double sigma, double *T, R *w, R *hist)
 {
     const int nv = dim + 1;
     const double inv_2s2 = 1.0 / (2.0 * sigma * sigma);
@@ -780,15 +811,16 @@ int somhip_launch_update_separable(hipStream_t st, const double *S, const float 
     }
     hipLaunchKernelGGL(update_sep_rows_kernel, dim3(n * np), dim3(256), lds1, st, S, m, n, nv, hex, inv_2s2, T);
     SOM_HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL(update_sep_cols_kernel, dim3(n), dim3(256), lds2, st, T, topo, m, n, dim, hex, inv_2s2, w, hist);
+    hipLaunchKernelGGL(update_sep_cols_kernel<R>, dim3(n), dim3(256), lds2, st, T, topo, m, n, dim, hex, inv_2s2, w, hist);
     SOM_HIP_TRY(hipGetLastError());
     return SOMHIP_OK;
 }
 
-int somhip_launch_divide(hipStream_t st, const double *acc, int mn, int dim, float *w, float *hist)
+template <class T>
+int somhip_launch_divide(hipStream_t st, const double *acc, int mn, int dim, T *w, T *hist)
 {
     const int64_t tot = (int64_t)mn * dim;
-    hipLaunchKernelGGL(divide_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, acc, 1, mn, dim, w, hist);
+    hipLaunchKernelGGL(divide_kernel<T>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, acc, 1, mn, dim, w, hist);
     SOM_HIP_TRY(hipGetLastError());
     return SOMHIP_OK;
 }
@@ -800,14 +832,15 @@ int somhip_iter_nsplit(int64_t n, int mn)
     return (int)std::min<int64_t>(s, 4096);
 }
 
-int somhip_launch_iter(hipStream_t st, const int32_t *bmu, const float *x, int dim, int dpad, const double *rho,
-                       int64_t n, int64_t cur, int64_t nvec, float sigma0, const float *topo, int mn, double *part,
+template <class T>
+int somhip_launch_iter(hipStream_t st, const int32_t *bmu, const T *x, int dim, int dpad, const double *rho,
+                       int64_t n, int64_t cur, int64_t nvec, T sigma0, const T *topo, int mn, double *part,
                        int nsplit, double *acc)
 {
     if (n <= 0) return SOMHIP_OK;
     const int64_t chunk = (n + nsplit - 1) / nsplit;
     dim3 g((mn + 255) / 256, nsplit), b(256);
-#define SOM_IT(D) hipLaunchKernelGGL(iter_partial_kernel<D>, g, b, 0, st, bmu, x, dim, rho, n, cur, nvec, sigma0, topo, mn, chunk, part)
+#define SOM_IT(D) hipLaunchKernelGGL((iter_partial_kernel<D, T>), g, b, 0, st, bmu, x, dim, rho, n, cur, nvec, sigma0, topo, mn, chunk, part)
     switch (dpad) {
         case 16: SOM_IT(16); break;
         case 32: SOM_IT(32); break;
@@ -826,15 +859,36 @@ int somhip_launch_iter(hipStream_t st, const int32_t *bmu, const float *x, int d
 // about 1024 samples (4 per thread) per workgroup: enough workgroups to keep the loads in flight
 int somhip_errors_nparts(int64_t n) { return (int)std::max<int64_t>(1, std::min<int64_t>(65536, (n + 1023) / 1024)); }
 
+template <class T>
 int somhip_launch_errors(hipStream_t st, const int32_t *c1, const int32_t *c2, const double *d1, const double *rho,
-                         int64_t n, const float *topo, double penalty, double *part, int nparts, double *errors)
+                         int64_t n, const T *topo, double penalty, double *part, int nparts, double *errors)
 {
     if (n <= 0) return SOMHIP_OK;
     const int64_t per_wg = (n + nparts - 1) / nparts;
-    hipLaunchKernelGGL(errors_partial_kernel, dim3(nparts), dim3(256), 0, st, c1, c2, d1, rho, n, topo, penalty, per_wg,
+    hipLaunchKernelGGL(errors_partial_kernel<T>, dim3(nparts), dim3(256), 0, st, c1, c2, d1, rho, n, topo, penalty, per_wg,
                        part);
     SOM_HIP_TRY(hipGetLastError());
     hipLaunchKernelGGL(errors_final_kernel, dim3(1), dim3(256), 0, st, part, nparts, errors);
     SOM_HIP_TRY(hipGetLastError());
     return SOMHIP_OK;
 }
+
+// FP32 (float32 SOMs) and FP64 (float64 SOMs) forms of the templated launchers
+#define SOM_TRAIN_LAUNCHERS(T)                                                                                  \
+    template int somhip_launch_absmax<T>(hipStream_t, const T *, int, int, const double *, int, int64_t,        \
+                                         unsigned int *, int);                                                  \
+    template int somhip_launch_scatter<T>(hipStream_t, const int32_t *, const T *, int, const double *, int,    \
+                                          int64_t, const int *, int, int, int, unsigned long long *, double *,  \
+                                          size_t, unsigned int *);                                              \
+    template int somhip_launch_update<T>(hipStream_t, const double *, const T *, int, int, double, double *,    \
+                                         int, T *, T *);                                                        \
+    template int somhip_launch_update_separable<T>(hipStream_t, const double *, const T *, int, int, int, int,  \
+                                                   double, double *, T *, T *);                                 \
+    template int somhip_launch_divide<T>(hipStream_t, const double *, int, int, T *, T *);                      \
+    template int somhip_launch_iter<T>(hipStream_t, const int32_t *, const T *, int, int, const double *,       \
+                                       int64_t, int64_t, int64_t, T, const T *, int, double *, int, double *);  \
+    template int somhip_launch_errors<T>(hipStream_t, const int32_t *, const int32_t *, const double *,         \
+                                         const double *, int64_t, const T *, double, double *, int, double *);
+SOM_TRAIN_LAUNCHERS(float)
+SOM_TRAIN_LAUNCHERS(double)
+#undef SOM_TRAIN_LAUNCHERS

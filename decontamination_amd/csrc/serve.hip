@@ -243,11 +243,11 @@ int somhip_server::finish(Request &R)
 {
     somhip_model *M = models[R.model];
     if (R.kind == SOMHIP_SERVE_ACTIVATION) {
-        int rc = somhip_launch_absmax(compute, nullptr, 0, 16, R.rho, 0, R.rho ? R.n : 0, R.bounds, ctx->num_cus);
+        int rc = somhip_launch_absmax(compute, NO_ROWS, 0, 16, R.rho, 0, R.rho ? R.n : 0, R.bounds, ctx->num_cus);
         if (!rc) rc = somhip_launch_shifts(compute, R.bounds, 0, 0, R.rho != nullptr, R.n, R.shifts);
         if (!rc) SOM_HIP_TRY(hipMemsetAsync(R.act, 0, sizeof(double) * M->mn, compute));
         if (!rc)
-            rc = somhip_launch_scatter(compute, R.c1, nullptr, 0, R.rho, 0, R.n, R.shifts, M->mn, 1, R.nwg, R.table,
+            rc = somhip_launch_scatter(compute, R.c1, NO_ROWS, 0, R.rho, 0, R.n, R.shifts, M->mn, 1, R.nwg, R.table,
                                        R.act, ctx->lds_per_block);
         if (rc) return rc;
         SOM_HIP_TRY(hipMemcpyAsync(R.out_host, R.act, sizeof(double) * M->mn, hipMemcpyDeviceToHost, compute));
@@ -548,6 +548,10 @@ int somhip_server_add_model(somhip_server *S, somhip_model *M, int *model_id)
     if (M->ctx != S->ctx) {
         somhip_set_error("model belongs to another context");
         return SOMHIP_ERR_ARG;
+    }
+    if (M->f64) {
+        somhip_set_error("serving takes float32 models");
+        return SOMHIP_ERR_UNSUPPORTED;
     }
     if (M->dpad > 32) {
         somhip_set_error("serving supports dim <= 32");

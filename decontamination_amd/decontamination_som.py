@@ -275,6 +275,16 @@ class Decontamination_SOM(object):
         self._som._require_gpu(enable_gpu)
         if self._winners is None:
             self._winners = self._som.get_winners(_materialise(self._footprint_systematics), enable_gpu=enable_gpu)
+        if self._som._f64():
+            # float64 maps: the reference's own sequence, two activation maps on the device, then gnd and
+            # gndm from them (decontamination_som.py:657-667)
+            self._catalog_activation_map = self._som.get_activation_map(
+                self._footprint_systematics, dataset_weights=self._corrected_galaxy_number_density,
+                enable_gpu=enable_gpu)
+            self._footprint_activation_map = self._som.get_activation_map(self._footprint_systematics,
+                                                                          enable_gpu=enable_gpu)
+            self._gnd, self._gndm = self._compute_gndm(self._catalog_activation_map, self._footprint_activation_map)
+            return
         (self._catalog_activation_map, self._footprint_activation_map, self._n_gal, self._n_pix, self._gnd,
          self._gndm) = compute_gndm(get_context(self._som._device), self._winners,
                                     _materialise(self._corrected_galaxy_number_density), self._som.m, self._som.n,

@@ -173,54 +173,79 @@ class Uploader:
             pass
 
 
-class Model:
-    """Weights (mn, dim) FP32 + topography (mn, 2) FP32 resident on the device."""
+def _opt(ptr):
+    return None if ptr is None else ctypes.c_void_p(ptr)
 
-    def __init__(self, ctx, m, n, dim, topology, topography):
-        topo = np.ascontiguousarray(topography, dtype=np.float32)
+
+class Model:
+    """Weights (mn, dim) + topography (mn, 2) resident on the device, FP32 (the default) or FP64
+    (``dtype=np.float64``: the float64 entry points, whose search is the exact FP64 scan and which
+    ignore the precision / arithmetic codes)."""
+
+    def __init__(self, ctx, m, n, dim, topology, topography, dtype=np.float32):
+        self.dtype = np.dtype(dtype).type
+        if self.dtype not in (np.float32, np.float64):
+            raise ValueError("device models hold float32 or float64 weights")
+        self.f64 = self.dtype is np.float64
+        topo = np.ascontiguousarray(topography, dtype=self.dtype)
         if topo.shape != (m * n, 2):
             raise ValueError("topography must have shape (m*n, 2)")
         h = ctypes.c_void_p()
         t = _lib.TOPOLOGY_SQUARE if topology == "square" else _lib.TOPOLOGY_HEXAGONAL
-        call("somhip_model_create", ctx.handle, int(m), int(n), int(dim), t, vp(topo), ctypes.byref(h))
+        call("somhip_model_create_f64" if self.f64 else "somhip_model_create", ctx.handle, int(m), int(n), int(dim),
+             t, vp(topo), ctypes.byref(h))
         self.ctx = ctx
         self.handle = h
         self.m, self.n, self.dim = int(m), int(n), int(dim)
         self.mn = self.m * self.n
         self.topology = topology
+        self._sfx = "_f64" if self.f64 else ""
 
     def set_weights(self, w):
+        fn = "somhip_model_set_weights" + self._sfx
         if isinstance(w, DeviceArray):
-            call("somhip_model_set_weights", self.handle, ctypes.c_void_p(w.ptr), 1)
+            call(fn, self.handle, ctypes.c_void_p(w.ptr), 1)
         elif isinstance(w, int):
-            call("somhip_model_set_weights", self.handle, ctypes.c_void_p(w), 1)
+            call(fn, self.handle, ctypes.c_void_p(w), 1)
         else:
-            arr = np.ascontiguousarray(w, dtype=np.float32).reshape(self.mn, self.dim)
-            call("somhip_model_set_weights", self.handle, vp(arr), 0)
+            arr = np.ascontiguousarray(w, dtype=self.dtype).reshape(self.mn, self.dim)
+            call(fn, self.handle, vp(arr), 0)
 
     def get_weights(self):
-        out = np.empty((self.mn, self.dim), np.float32)
-        call("somhip_model_get_weights", self.handle, vp(out))
+        out = np.empty((self.mn, self.dim), self.dtype)
+        call("somhip_model_get_weights" + self._sfx, self.handle, vp(out))
         return out
 
     def weights_ptr(self):
+        if self.f64:
+            raise ValueError("float64 models have no FP32 weights pointer")
         p = ctypes.c_void_p()
         call("somhip_model_weights_ptr", self.handle, ctypes.byref(p))
         return p.value
 
     # -- inference (device pointers in, device pointers out) ------------------------------------
     def find_bmus(self, x_ptr, n, out_ptr, precision, arithmetic):
-        call("somhip_find_bmus", self.handle, ctypes.c_void_p(x_ptr), int(n), precision, arithmetic,
-             ctypes.c_void_p(out_ptr))
+        if self.f64:
+            call("somhip_find_bmus_f64", self.handle, ctypes.c_void_p(x_ptr), int(n), ctypes.c_void_p(out_ptr))
+        else:
+            call("somhip_find_bmus", self.handle, ctypes.c_void_p(x_ptr), int(n), precision, arithmetic,
+                 ctypes.c_void_p(out_ptr))
 
     def count_bmus(self, x_ptr, rho_ptr, n, result_ptr, precision, arithmetic):
-        call("somhip_count_bmus", self.handle, ctypes.c_void_p(x_ptr), None if rho_ptr is None else ctypes.c_void_p(rho_ptr),
-             int(n), precision, arithmetic, ctypes.c_void_p(result_ptr))
+        if self.f64:
+            call("somhip_count_bmus_f64", self.handle, ctypes.c_void_p(x_ptr), _opt(rho_ptr), int(n),
+                 ctypes.c_void_p(result_ptr))
+        else:
+            call("somhip_count_bmus", self.handle, ctypes.c_void_p(x_ptr), _opt(rho_ptr), int(n), precision,
+                 arithmetic, ctypes.c_void_p(result_ptr))
 
     def compute_errors(self, x_ptr, rho_ptr, n, errors_ptr, precision, arithmetic):
-        call("somhip_compute_errors", self.handle, ctypes.c_void_p(x_ptr),
-             None if rho_ptr is None else ctypes.c_void_p(rho_ptr), int(n), precision, arithmetic,
-             ctypes.c_void_p(errors_ptr))
+        if self.f64:
+            call("somhip_compute_errors_f64", self.handle, ctypes.c_void_p(x_ptr), _opt(rho_ptr), int(n),
+                 ctypes.c_void_p(errors_ptr))
+        else:
+            call("somhip_compute_errors", self.handle, ctypes.c_void_p(x_ptr), _opt(rho_ptr), int(n), precision,
+                 arithmetic, ctypes.c_void_p(errors_ptr))
 
     def close(self):
         if self.handle:
@@ -246,22 +271,33 @@ class Accumulator:
     def reset(self):
         call("somhip_accum_reset", self.handle)
 
+    # sigma0 is passed in the model's type (the reference casts it to dtype: som_batch.py:315)
     def epoch_accumulate(self, x_ptr, rho_ptr, n, precision, arithmetic, errors_ptr=None):
-        call("somhip_batch_epoch_accumulate", self.handle, ctypes.c_void_p(x_ptr),
-             None if rho_ptr is None else ctypes.c_void_p(rho_ptr), int(n), precision, arithmetic,
-             None if errors_ptr is None else ctypes.c_void_p(errors_ptr))
+        if self.model.f64:
+            call("somhip_batch_epoch_accumulate_f64", self.handle, ctypes.c_void_p(x_ptr), _opt(rho_ptr), int(n),
+                 _opt(errors_ptr))
+        else:
+            call("somhip_batch_epoch_accumulate", self.handle, ctypes.c_void_p(x_ptr), _opt(rho_ptr), int(n),
+                 precision, arithmetic, _opt(errors_ptr))
 
     def epoch_update(self, cur_epoch, n_epochs, sigma0, history_ptr=None):
-        call("somhip_batch_epoch_update", self.handle, int(cur_epoch), int(n_epochs), float(np.float32(sigma0)),
-             None if history_ptr is None else ctypes.c_void_p(history_ptr))
+        if self.model.f64:
+            call("somhip_batch_epoch_update_f64", self.handle, int(cur_epoch), int(n_epochs),
+                 float(np.float64(sigma0)), _opt(history_ptr))
+        else:
+            call("somhip_batch_epoch_update", self.handle, int(cur_epoch), int(n_epochs),
+                 float(np.float32(sigma0)), _opt(history_ptr))
 
     def iter_accumulate(self, x_ptr, rho_ptr, n, cur_vector, n_vectors, sigma0, precision, arithmetic):
-        call("somhip_batch_iter_accumulate", self.handle, ctypes.c_void_p(x_ptr),
-             None if rho_ptr is None else ctypes.c_void_p(rho_ptr), int(n), int(cur_vector), int(n_vectors),
-             float(np.float32(sigma0)), precision, arithmetic)
+        if self.model.f64:
+            call("somhip_batch_iter_accumulate_f64", self.handle, ctypes.c_void_p(x_ptr), _opt(rho_ptr), int(n),
+                 int(cur_vector), int(n_vectors), float(np.float64(sigma0)))
+        else:
+            call("somhip_batch_iter_accumulate", self.handle, ctypes.c_void_p(x_ptr), _opt(rho_ptr), int(n),
+                 int(cur_vector), int(n_vectors), float(np.float32(sigma0)), precision, arithmetic)
 
     def iter_update(self, history_ptr=None):
-        call("somhip_batch_iter_update", self.handle, None if history_ptr is None else ctypes.c_void_p(history_ptr))
+        call("somhip_batch_iter_update" + ("_f64" if self.model.f64 else ""), self.handle, _opt(history_ptr))
 
     def close(self):
         if self.handle:

@@ -50,6 +50,9 @@ class SOMServer:
     def __init__(self, soms, memory_budget=64 << 30, arithmetic="ref32", device=0):
         if arithmetic not in _lib.ARITHMETICS:
             raise ValueError(f"invalid arithmetic `{arithmetic}`")
+        soms = list(soms)
+        if any(som.dtype is not np.float32 for som in soms):
+            raise NotImplementedError("serving supports float32 SOMs")
         self.ctx = get_context(device)
         h = ctypes.c_void_p()
         call("somhip_server_create", self.ctx.handle, int(memory_budget), _lib.ARITHMETICS[arithmetic],
@@ -58,8 +61,6 @@ class SOMServer:
         self.soms = list(soms)
         self.models = []
         for som in self.soms:
-            if som.dtype is not np.float32:
-                raise NotImplementedError("serving supports float32 SOMs")
             mdl = Model(self.ctx, som.m, som.n, som.dim, som.topology, som._topography)
             mdl.set_weights(som.weights)
             mid = ctypes.c_int()

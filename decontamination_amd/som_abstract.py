@@ -11,6 +11,10 @@ types and dtypes. Additive keyword-only options select the BMU search precision:
   arithmetic 'ref32' (default) the reference's distance arithmetic (FP32 difference and square, FP64
              sum); 'fp64' FP64 throughout.
 
+``dtype=np.float64`` keeps weights, topography, data and sums in FP64 on the device, as the
+reference's float64 SOMs do; their search is the exact FP64 scan (precision 'exact', arithmetic
+'fp64', the only values such a SOM takes). float32 and float64 are the dtypes the GPU path supports.
+
 This package is the GPU path only. ``enable_gpu=False`` raises: the CPU path stays the reference's
 own. ``threads_per_blocks`` is accepted for signature compatibility and ignored (launch shapes are
 fixed by the kernels).
@@ -56,7 +60,7 @@ class SOM_Abstract(object):
     __MODE__ = 'abstract'
 
     def __init__(self, m: int, n: int, dim: int, dtype=np.float32, topology: typing.Optional[str] = None, *,
-                 precision: str = 'bf16_rerank', arithmetic: str = 'ref32', device: int = 0):
+                 precision: typing.Optional[str] = None, arithmetic: typing.Optional[str] = None, device: int = 0):
         self._m = int(m)
         self._n = int(n)
         self._dim = int(dim)
@@ -69,8 +73,10 @@ class SOM_Abstract(object):
         self._topographic_errors = np.empty(0, dtype=np.float64)
         self._header_extra = {'mode': '__MODE__'}
         self._dataset_extra = {}
-        self._precision = self._check_precision(precision)
-        self._arithmetic = self._check_arithmetic(arithmetic)
+        f64 = self._f64()
+        self._precision = self._check_precision(precision or ('exact' if f64 else 'bf16_rerank'))
+        self._arithmetic = self._check_arithmetic(arithmetic or ('fp64' if f64 else 'ref32'))
+        self._check_f64_search(self._precision, self._arithmetic)
         self._device = int(device)
         self._dev_model = None
         self._staging = None
@@ -88,9 +94,22 @@ class SOM_Abstract(object):
             raise ValueError(f'Invalid arithmetic `{a}`, expected one of {sorted(_lib.ARITHMETICS)}')
         return a
 
+    def _f64(self):
+        """True for a float64 SOM."""
+        try:
+            return np.dtype(self._dtype) == np.float64
+        except TypeError:
+            return False
+
+    def _check_f64_search(self, p, a):
+        if self._f64() and (p != 'exact' or a != 'fp64'):
+            raise ValueError('float64 SOMs search with the exact FP64 distance (precision `exact`, arithmetic '
+                             f'`fp64`), not precision `{p}` with arithmetic `{a}`')
+
     def _prec_codes(self, precision=None, arithmetic=None):
         p = self._check_precision(precision or self._precision)
         a = self._check_arithmetic(arithmetic or self._arithmetic)
+        self._check_f64_search(p, a)
         return _lib.PRECISIONS[p], _lib.ARITHMETICS[a]
 
     @property
@@ -282,16 +301,25 @@ class SOM_Abstract(object):
         if not enable_gpu:
             raise NotImplementedError('decontamination_amd implements the GPU path only; the CPU path is the '
                                       "reference's own (decontamination.SOM_*)")
-        if self._dtype is not np.float32:
-            raise NotImplementedError('the GPU path supports dtype=np.float32')
+        try:
+            dt = np.dtype(self._dtype)
+        except TypeError:
+            dt = None
+        if dt not in (np.dtype(np.float32), np.dtype(np.float64)):
+            raise NotImplementedError('the GPU path supports dtype=np.float32 and dtype=np.float64')
 
     def _ctx(self):
         return get_context(self._device)
 
+    def _device_dtype(self):
+        """Type of the device weights and history: float64 for float64 SOMs, else float32."""
+        return np.float64 if self._f64() else np.float32
+
     def _device_model(self, upload=True):
         """Device model holding the current host weights."""
         if self._dev_model is None or self._dev_model.handle is None:
-            self._dev_model = Model(self._ctx(), self._m, self._n, self._dim, self._topology, self._topography)
+            self._dev_model = Model(self._ctx(), self._m, self._n, self._dim, self._topology, self._topography,
+                                    dtype=self._device_dtype())
         if upload:
             self._dev_model.set_weights(self._weights)
         return self._dev_model

@@ -74,7 +74,7 @@ class SOM_Batch(SOM_Abstract):
             raise ValueError('Invalid training method, specify either `n_epochs` or `n_vectors`')
         self._require_gpu(enable_gpu)
         prec, arith = self._prec_codes(precision, arithmetic)
-        sigma0 = np.float32(self._sigma)
+        sigma0 = self._device_dtype()(self._sigma)  # cast to the dtype, as the reference does (som_batch.py:315)
         mn = self._m * self._n
         model = self._device_model()
         ctx = self._ctx()
@@ -107,7 +107,7 @@ class SOM_Batch(SOM_Abstract):
                       stop_t, show_progress_bar):
         mn, dim = self._m * self._n, self._dim
         early_stop = stop_q > -1.0e6 or stop_t > -1.0e6
-        history = DeviceArray.empty(ctx, (n_epochs, mn, dim), np.float32)
+        history = DeviceArray.empty(ctx, (n_epochs, mn, dim), self._device_dtype())
         errors = DeviceArray.zeros(ctx, (n_epochs, 3), np.float64)
         resident = not callable(dataset) and (dataset_weights is None or not callable(dataset_weights))
         res_x = res_r = None
@@ -187,7 +187,7 @@ class SOM_Batch(SOM_Abstract):
                 bar.update(count)
             if cur >= n_vectors:
                 break
-        history = DeviceArray.empty(ctx, (mn, dim), np.float32)
+        history = DeviceArray.empty(ctx, (mn, dim), self._device_dtype())
         acc.iter_update(history.ptr)
         self._history[0] = history.copy_to_host()
         self._weights[:] = self._history[0]

@@ -214,6 +214,32 @@ int somhip_batch_iter_accumulate(somhip_accum *acc, const float *x, const double
 int somhip_batch_iter_update(somhip_accum *acc, float *history_out);
 
 /* ------------------------------------------------------------------------------------------------
+ * float64 SOMs: the reference's dtype=np.float64 (som_abstract.py:46-68, som_batch.py:289-291,
+ * 311-315), where weights, topography, data (`vectors.astype(dtype)`), sigma and the sums are FP64.
+ * A float64 model holds FP64 weights and topography; its entry points take FP64 device rows and
+ * search with the exact FP64 scan (the reference's square_distance_xpu on float64 operands:
+ * FP64 difference, square and sequential sum), so they have no precision/arithmetic arguments.
+ * The per-BMU sums are the same exact fixed-point sums of the FP64 products rho*x; the density is not
+ * rounded. The update computes the neighbourhood from FP64 topography differences and writes FP64
+ * weights. Each replaces the float32 entry point of the same name (same outputs and errors); the
+ * float32 entry points reject float64 models and the reverse, and serving takes float32 models only.
+ * ------------------------------------------------------------------------------------------------ */
+int somhip_model_create_f64(somhip_ctx *ctx, int m, int n, int dim, int topology, const double *topography_host,
+                            somhip_model **out);
+int somhip_model_set_weights_f64(somhip_model *model, const double *w, int on_device);
+int somhip_model_get_weights_f64(somhip_model *model, double *w_host);
+int somhip_find_bmus_f64(somhip_model *model, const double *x, int64_t n, int32_t *out);
+int somhip_count_bmus_f64(somhip_model *model, const double *x, const double *rho, int64_t n, double *result);
+int somhip_compute_errors_f64(somhip_model *model, const double *x, const double *rho, int64_t n, double *errors);
+int somhip_batch_epoch_accumulate_f64(somhip_accum *acc, const double *x, const double *rho, int64_t n,
+                                      double *errors);
+int somhip_batch_epoch_update_f64(somhip_accum *acc, int64_t cur_epoch, int64_t n_epochs, double sigma0,
+                                  double *history_out);
+int somhip_batch_iter_accumulate_f64(somhip_accum *acc, const double *x, const double *rho, int64_t n,
+                                     int64_t cur_vector, int64_t n_vectors, double sigma0);
+int somhip_batch_iter_update_f64(somhip_accum *acc, double *history_out);
+
+/* ------------------------------------------------------------------------------------------------
  * Serving. No counterpart in the reference (it has no serving loop); built on the same kernels as
  * somhip_find_bmus / somhip_count_bmus with PREC_BF16_RERANK.
  *

@@ -257,6 +257,102 @@ def factorised_epoch_fp64(W, topo, X, rho, sigma, bmu=None, out_dtype=np.float32
 
 
 # ---------------------------------------------------------------------------------------------------
+# float64 SOMs (dtype=np.float64): the same rules with data, weights, topography, density and sums in
+# FP64. Pure numpy over small inputs; pinned in tests/test_oracle.py against the C FP64 kernels on inputs
+# that FP32 represents exactly, where both must agree.
+# ---------------------------------------------------------------------------------------------------
+
+def sqd_f64(A, B):
+    """square_distance_xpu (algo/__init__.py:89-98) on float64 operands, broadcast over leading axes:
+    sum over k, in k order, of fl(fl(a_k - b_k)^2)."""
+    A = np.asarray(A, np.float64); B = np.asarray(B, np.float64)
+    r = np.zeros(np.broadcast_shapes(A.shape[:-1], B.shape[:-1]))
+    for k in range(A.shape[-1]):
+        t = A[..., k] - B[..., k]
+        r = r + t * t
+    return r
+
+
+def top2_f64(W, X, block=256):
+    """(c1, c2, d1) of the _compute_errors_xpu top-2 rule (som_abstract.py:816-853) on float64 operands:
+    NaN or >= 1e99 distances never enter, ties keep the lower index, -1 / NaN when absent."""
+    W = np.asarray(W, np.float64); X = np.asarray(X, np.float64)
+    N, mn = X.shape[0], W.shape[0]
+    c1 = np.full(N, -1, np.int32); c2 = np.full(N, -1, np.int32); d1 = np.full(N, np.nan)
+    for s in range(0, N, block):
+        d = sqd_f64(X[s:s + block, None, :], W[None, :, :])
+        d = np.where(np.isnan(d) | (d >= 1.0e99), np.inf, d)
+        o = np.argsort(d, axis=1, kind="stable")  # (distance, index) order
+        first = np.take_along_axis(d, o[:, :1], 1)[:, 0]
+        ok1 = first < np.inf
+        c1[s:s + block] = np.where(ok1, o[:, 0], -1)
+        d1[s:s + block] = np.where(ok1, first, np.nan)
+        if mn > 1:
+            second = np.take_along_axis(d, o[:, 1:2], 1)[:, 0]
+            c2[s:s + block] = np.where(second < np.inf, o[:, 1], -1)
+    return c1, c2, d1
+
+
+def find_bmus_f64(W, X):
+    """_find_bmu_xpu (som_abstract.py:764-783) on float64 operands: first minimum, 0 when none."""
+    c1, _, _ = top2_f64(W, X)
+    return np.where(c1 >= 0, c1, 0).astype(np.int32)
+
+
+def activation_map_f64(W, X, rho=None):
+    """_count_bmus_kernel (som_abstract.py:699-732): result[bmu(x_i)] += rho_i for rho_i > 0."""
+    b = find_bmus_f64(W, X)
+    r = np.ones(X.shape[0]) if rho is None else np.asarray(rho, np.float64)
+    keep = r > 0.0
+    return np.bincount(b[keep], weights=r[keep], minlength=np.asarray(W).shape[0])
+
+
+def errors_f64(W, topo, X, rho=None, topology="hexagonal"):
+    """compute_errors (som_abstract.py:547-601, 816-878) of a float64 SOM: (QE, TE), the topography
+    distance of the two BMUs in FP64 against r^2 = 2 (square) or 1 (hexagonal)."""
+    c1, c2, d1 = top2_f64(W, X)
+    r = np.ones(X.shape[0]) if rho is None else np.asarray(rho, np.float64)
+    ok = (r > 0.0) & (c1 >= 0) & (c2 >= 0) & (c1 != c2)
+    t = np.asarray(topo, np.float64)
+    far = sqd_f64(t[c1[ok]], t[c2[ok]]) > (2.0 if topology == "square" else 1.0)
+    s = r[ok].sum()
+    if W.shape[0] < 2 or not s > 0.0:
+        return np.nan, np.nan
+    return float((r[ok] * np.sqrt(d1[ok])).sum() / s), float(r[ok][far].sum() / s)
+
+
+def _accumulate_f64(W, topo, X, rho, inv_2s2):
+    """_train_step2_xpu (som_batch.py:509-566) on float64 operands: every sample with rho > 0 and a BMU
+    adds h_j = rho exp(-|t_j - t_bmu|^2 inv_2s2) x to num_j and h_j to den_j (inv_2s2 per sample)."""
+    c1, _, _ = top2_f64(W, X)
+    r = np.ones(X.shape[0]) if rho is None else np.asarray(rho, np.float64)
+    keep = (r > 0.0) & (c1 >= 0)
+    t = np.asarray(topo, np.float64)
+    inv = np.broadcast_to(np.asarray(inv_2s2, np.float64), r.shape)[keep]
+    h = r[keep][None, :] * np.exp(-sqd_f64(t[:, None, :], t[c1[keep]][None, :, :]) * inv[None, :])
+    return h @ np.asarray(X, np.float64)[keep], h.sum(1)
+
+
+def _divide_f64(num, den):
+    """w = num / den where den > 1e-12, else 0 (som_batch.py:354-359), in FP64."""
+    return np.where(den[:, None] > FLOAT_EPS, num / np.where(den[:, None] > FLOAT_EPS, den[:, None], 1.0), 0.0)
+
+
+def epoch_f64(W, topo, X, rho, sigma0, e, n_epochs):
+    """One epoch-mode epoch of a float64 SOM: sigma = sigma0 / (1 + 2 e / E) >= 1e-12 (som_batch.py:107-110)."""
+    s = max(float(sigma0) * (1.0 / (1.0 + 2.0 * e / n_epochs)), FLOAT_EPS)
+    return _divide_f64(*_accumulate_f64(W, topo, X, rho, 1.0 / (2.0 * s * s)))
+
+
+def iter_f64(W, topo, X, rho, sigma0, cur, n_vectors):
+    """Vector-count mode of a float64 SOM over X (som_batch.py:154-210, 386-486): sigma_i =
+    sigma0 / (1 + 2 (cur + i) / n_vectors) >= 1e-12 per sample, then the division."""
+    i = np.arange(np.asarray(X).shape[0])
+    s = np.maximum(float(sigma0) * (1.0 / (1.0 + 2.0 * (cur + i) / n_vectors)), FLOAT_EPS)
+    return _divide_f64(*_accumulate_f64(W, topo, X, rho, 1.0 / (2.0 * s * s)))
+
+
+# ---------------------------------------------------------------------------------------------------
 # Covariance / SOM_PCA / galaxy-number-density restatements
 # ---------------------------------------------------------------------------------------------------
 def covariance(dim, dataset, dataset_weights=None, ddof=1):

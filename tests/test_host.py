@@ -24,6 +24,24 @@ def test_constructor_defaults_follow_reference():
     assert som.precision == "bf16_rerank" and som.arithmetic == "ref32"
 
 
+def test_float64_soms_search_exactly_in_fp64():
+    """dtype=np.float64 (reference som_abstract.py:46): FP64 weights and topography on the host, and the
+    exact FP64 search as the only precision; other dtypes are refused before any device call."""
+    som = SOM_Batch(3, 4, 2, dtype=np.float64)
+    assert som.dtype is np.float64 and som.weights.dtype == np.float64 and som._topography.dtype == np.float64
+    assert som.precision == "exact" and som.arithmetic == "fp64"
+    assert som._topography[3, 1] == 3 * 0.8660254037844386
+    assert SOM_Batch(3, 4, 2, dtype=np.float64, precision="exact", arithmetic="fp64").precision == "exact"
+    with pytest.raises(ValueError):
+        SOM_Batch(3, 4, 2, dtype=np.float64, precision="bf16_rerank")
+    with pytest.raises(ValueError):
+        SOM_Batch(3, 4, 2, dtype=np.float64, arithmetic="ref32")
+    with pytest.raises(ValueError):
+        som.get_winners(np.zeros((2, 2)), precision="fp32")
+    with pytest.raises(NotImplementedError):
+        SOM_Batch(2, 2, 2, dtype=np.int32).get_winners(np.zeros((2, 2), np.int32))
+
+
 def test_topography_closed_forms():
     sq = SOM_Batch(3, 4, 2, topology="square")._topography
     assert np.array_equal(sq, np.array([[i, j] for i in range(3) for j in range(4)], np.float32))

@@ -300,3 +300,71 @@ def test_trajectory_fp64_golden_and_multi_epoch():
     Ws, hist, qes, _ = R.train(Wh0, 4, 5, Xh, n_epochs=3, sigma=2.5, use_best_epoch=False, mode=FP64)
     assert np.allclose(Wt, Ws, rtol=1e-6, atol=1e-6)
     assert abs(qet - qes[-1]) <= 1e-5 * qes[-1]  # the QE of the final weights
+
+
+# ---------------------------------------------------------------- float64 SOMs ---------------
+
+def test_float64_oracle_equals_c_fp64_on_fp32_inputs():
+    """On data, weights and topography that FP32 represents exactly (square grid: integer coordinates),
+    the float64 restatements and the C FP64 kernels compute the same quantities: top-2 bit for bit,
+    errors, one epoch and the vector-count mode to FP64 rounding."""
+    rng = np.random.default_rng(31)
+    m, n, dim = 6, 7, 8
+    W = rng.standard_normal((m * n, dim)).astype(np.float32)
+    X = rng.standard_normal((700, dim)).astype(np.float32)
+    W[30] = W[3]
+    X[:4] = W[3]
+    X[7, 2] = np.nan
+    X[9] = np.nan
+    rho = rng.random(700)
+    rho[::11] = 0.0
+    rho[1::13] = -1.0
+    topo = R.topography(m, n, "square")
+    c1, c2, d1 = R.top2_f64(W, X)
+    e1, e2, ed = R.top2_omp(W, X, FP64)
+    assert np.array_equal(c1, e1) and np.array_equal(c2, e2) and np.array_equal(d1, ed, equal_nan=True)
+    assert c1[0] == 3 and c2[0] == 30 and c1[9] == -1  # tie: lower index first; NaN row: none
+    assert np.array_equal(R.find_bmus_f64(W, X), R.find_bmus(W, X, FP64))
+    assert np.allclose(R.activation_map_f64(W, X, rho), R.activation_map(W, X, rho, FP64), rtol=1e-12, atol=0)
+    assert np.allclose(R.errors_f64(W, topo, X, rho, "square"), R.compute_errors(W, topo, X, rho, "square", FP64),
+                       rtol=1e-12, atol=0)
+    sigma0 = np.float32(3.0)
+    ok = (rho > 0) & (c1 >= 0)
+    want, _, _, _ = R.factorised_epoch_fp64(W, topo, X[ok], rho[ok], R.sigma_at(sigma0, 1, 5), bmu=c1[ok],
+                                            out_dtype=np.float64)
+    assert np.allclose(R.epoch_f64(W, topo, X, rho, sigma0, 1, 5), want, rtol=1e-12, atol=1e-14)
+    num, den = R.epoch_accumulate(W, topo, X, rho, 10, 5000, sigma0, FP64, True)
+    want = np.where(den[:, None] > 1e-12, num / np.where(den[:, None] > 1e-12, den[:, None], 1.0), 0.0)
+    assert np.allclose(R.iter_f64(W, topo, X, rho, sigma0, 10, 5000), want, rtol=1e-12, atol=1e-14)
+
+
+def test_float64_oracle_brute_force_and_fp64_topography():
+    """The float64 top-2 on values FP32 cannot hold equals a per-neuron Python loop of the reference rule;
+    and the hexagonal threshold uses FP64 topography: the grid neighbours (0, 2) and (0, 3) lie at
+    1.0000000000000004 > 1 in FP64 (a topographic error) but at 0.99999988 <= 1 with the FP32
+    topography of float32 SOMs (none)."""
+    rng = np.random.default_rng(32)
+    W = rng.standard_normal((9, 5)) / 3.0
+    X = rng.standard_normal((40, 5)) / 7.0
+    assert not np.array_equal(X.astype(np.float32).astype(np.float64), X)
+    c1, c2, d1 = R.top2_f64(W, X)
+    for i in range(X.shape[0]):
+        best = []
+        for j in range(W.shape[0]):
+            d = 0.0
+            for k in range(W.shape[1]):
+                t = X[i, k] - W[j, k]
+                d += t * t
+            best.append((d, j))
+        best.sort()
+        assert (c1[i], c2[i], d1[i]) == (best[0][1], best[1][1], best[0][0])
+    t64 = R.topography(2, 4, "hexagonal", np.float64)
+    assert R.sqd_f64(t64[2], t64[3]) == 1.0000000000000004
+    Wn = np.full((8, 2), 100.0)
+    Wn[2] = [0.0, 0.0]
+    Wn[3] = [0.1, 0.0]
+    Xs = np.array([[0.04, 0.0]])
+    assert R.errors_f64(Wn, t64, Xs, None, "hexagonal") == (0.04, 1.0)
+    _, te32 = R.compute_errors(Wn.astype(np.float32), R.topography(2, 4), Xs.astype(np.float32), None, "hexagonal",
+                               FP64)
+    assert te32 == 0.0
