@@ -51,7 +51,8 @@ def compute_gndm(ctx, winners: np.ndarray, corrected_density: np.ndarray, m: int
     winners and the coverage-corrected galaxy number density, as Decontamination_SOM.compute_gndm
     produces them (decontamination_som.py:600-667)."""
     if dtype is not np.float32:
-        raise NotImplementedError('the GPU path supports dtype=np.float32')
+        raise NotImplementedError('the fused gndm kernel writes float32 maps; float64 SOMs take '
+                                  'Decontamination_SOM.compute_gndm, which builds the two activation maps')
     mn = m * n
     w = np.ascontiguousarray(winners, dtype=np.int32).reshape(-1)
     rho = np.ascontiguousarray(corrected_density, dtype=np.float64).reshape(-1)
