@@ -400,6 +400,155 @@ bool closed_form_grid(const T *topo, int m, int n, int topology)
     return true;
 }
 
+// The model's weights and topography in T: float for float32 models, double for float64 models.
+template <class T> T *model_w(somhip_model *M);
+template <> float *model_w<float>(somhip_model *M) { return M->w; }
+template <> double *model_w<double>(somhip_model *M) { return M->w64; }
+template <class T> const T *model_topo(const somhip_model *M);
+template <> const float *model_topo<float>(const somhip_model *M) { return M->topo; }
+template <> const double *model_topo<double>(const somhip_model *M) { return M->topo64; }
+
+// The BMU search: FP32 rows with the requested precision and arithmetic, FP64 rows (float64 models)
+// with the exact FP64 scan.
+int search(somhip_model *M, const float *x, int64_t n, int precision, int arithmetic, int need_c2, int default_index,
+           int32_t *c1, int32_t *c2, double *d1)
+{
+    return somhip_launch_bmu(M, x, n, precision, arithmetic, need_c2, default_index, c1, c2, d1, nullptr);
+}
+int search(somhip_model *M, const double *x, int64_t n, int, int, int need_c2, int default_index, int32_t *c1,
+           int32_t *c2, double *d1)
+{
+    return somhip_launch_bmu_f64(M, x, n, need_c2, default_index, c1, c2, d1);
+}
+
+// Bodies shared by the float32 and float64 entry points (model and precision already checked).
+template <class T>
+int count_bmus_t(somhip_model *M, const T *x, const double *rho, int64_t n, int precision, int arithmetic,
+                 double *result)
+{
+    if (n < 0 || (n > 0 && (!x || !result))) {
+        somhip_set_error("invalid x/result/n");
+        return SOMHIP_ERR_ARG;
+    }
+    if (n == 0) return SOMHIP_OK;
+    somhip_ctx *C = M->ctx;
+    Workspace &W = C->ws;
+    const int nwg = somhip_scatter_nwg(n, M->mn, 1, C->num_cus, SCATTER_TABLE_BUDGET);
+    int rc;
+    if ((rc = W.c1.ensure(sizeof(int32_t) * n)) || (rc = W.bounds.ensure(256)) || (rc = W.shifts.ensure(256)) ||
+        (rc = W.table.ensure((size_t)nwg * M->mn * 8)))
+        return rc;
+    int32_t *c1 = (int32_t *)W.c1.p;
+    if ((rc = search(M, x, n, precision, arithmetic, 0, 0, c1, nullptr, nullptr))) return rc;
+    {
+        TimerScope ts(C, TK_ABSMAX);
+        if ((rc = somhip_launch_absmax(C->stream, NO_ROWS, 0, 16, rho, 0, rho ? n : 0, (unsigned int *)W.bounds.p,
+                                       C->num_cus)))
+            return rc;
+        if ((rc = somhip_launch_shifts(C->stream, (const unsigned int *)W.bounds.p, 0, 0, rho != nullptr, n,
+                                       (int *)W.shifts.p)))
+            return rc;
+    }
+    TimerScope ts(C, TK_SCATTER);
+    return somhip_launch_scatter(C->stream, c1, NO_ROWS, 0, rho, 0, n, (const int *)W.shifts.p, M->mn, 1, nwg,
+                                 (unsigned long long *)W.table.p, result, C->lds_per_block);
+}
+
+template <class T>
+int compute_errors_t(somhip_model *M, const T *x, const double *rho, int64_t n, int precision, int arithmetic,
+                     double *errors)
+{
+    if (n < 0 || (n > 0 && (!x || !errors))) {
+        somhip_set_error("invalid x/errors/n");
+        return SOMHIP_ERR_ARG;
+    }
+    if (n == 0) return SOMHIP_OK;
+    somhip_ctx *C = M->ctx;
+    Workspace &W = C->ws;
+    const int nparts = somhip_errors_nparts(n);
+    int rc;
+    if ((rc = W.c1.ensure(sizeof(int32_t) * n)) || (rc = W.c2.ensure(sizeof(int32_t) * n)) ||
+        (rc = W.d1.ensure(sizeof(double) * n)) || (rc = W.part.ensure(sizeof(double) * 3 * nparts)))
+        return rc;
+    int32_t *c1 = (int32_t *)W.c1.p, *c2 = (int32_t *)W.c2.p;
+    double *d1 = (double *)W.d1.p;
+    if ((rc = search(M, x, n, precision, arithmetic, 1, -1, c1, c2, d1))) return rc;
+    const double penalty = M->topology == SOMHIP_TOPOLOGY_SQUARE ? 2.0 : 1.0;
+    TimerScope ts(C, TK_ERRORS);
+    return somhip_launch_errors(C->stream, c1, c2, d1, rho, n, model_topo<T>(M), penalty, (double *)W.part.p, nparts,
+                                errors);
+}
+
+template <class T>
+int epoch_update_t(somhip_accum *A, int64_t cur_epoch, int64_t n_epochs, double sigma0, T *history_out)
+{
+    somhip_model *M = A->model;
+    if (A->mode != 1) {
+        somhip_set_error("epoch update without epoch accumulation");
+        return SOMHIP_ERR_STATE;
+    }
+    if (n_epochs <= 0 || cur_epoch < 0) {
+        somhip_set_error("invalid epoch counters");
+        return SOMHIP_ERR_ARG;
+    }
+    const double sigma = epoch_sigma(sigma0, cur_epoch, n_epochs);
+    M->prepared = false;  // the derived data of the FP32 search follow the new weights
+    int rc;
+    TimerScope ts(M->ctx, TK_UPDATE);
+    if (M->separable && somhip_update_separable_fits(M->m, M->n, M->dim)) {
+        if ((rc = A->upd.ensure(sizeof(double) * 2 * (size_t)M->mn * (M->dim + 1)))) return rc;
+        return somhip_launch_update_separable(M->ctx->stream, A->acc, model_topo<T>(M), M->m, M->n,
+                                              M->topology == SOMHIP_TOPOLOGY_HEXAGONAL ? 1 : 0, M->dim, sigma,
+                                              (double *)A->upd.p, model_w<T>(M), history_out);
+    }
+    const int nsplit = somhip_update_nsplit(M->mn);
+    if ((rc = A->upd.ensure(sizeof(double) * (size_t)nsplit * M->mn * (M->dim + 1)))) return rc;
+    return somhip_launch_update(M->ctx->stream, A->acc, model_topo<T>(M), M->mn, M->dim, sigma, (double *)A->upd.p,
+                                nsplit, model_w<T>(M), history_out);
+}
+
+template <class T>
+int iter_accumulate_t(somhip_accum *A, const T *x, const double *rho, int64_t n, int64_t cur_vector,
+                      int64_t n_vectors, T sigma0, int precision, int arithmetic)
+{
+    somhip_model *M = A->model;
+    if (A->mode == 1) {
+        somhip_set_error("accumulator holds epoch sums; reset it first");
+        return SOMHIP_ERR_STATE;
+    }
+    if (n < 0 || (n > 0 && !x) || n_vectors <= 0 || cur_vector < 0) {
+        somhip_set_error("invalid x/n/vector counters");
+        return SOMHIP_ERR_ARG;
+    }
+    A->mode = 2;
+    if (n == 0) return SOMHIP_OK;
+    somhip_ctx *C = M->ctx;
+    Workspace &W = A->ws;
+    const int nsplit = somhip_iter_nsplit(n, M->mn);
+    int rc;
+    if ((rc = W.c1.ensure(sizeof(int32_t) * n)) ||
+        (rc = A->upd.ensure(sizeof(double) * (size_t)nsplit * M->mn * (M->dim + 1))))
+        return rc;
+    int32_t *c1 = (int32_t *)W.c1.p;
+    W.c1_x = nullptr;  // c1 no longer holds the epoch path's BMUs
+    if ((rc = search(M, x, n, precision, arithmetic, 0, -1, c1, nullptr, nullptr))) return rc;
+    TimerScope ts(C, TK_ITER);
+    return somhip_launch_iter(C->stream, c1, x, M->dim, M->dpad, rho, n, cur_vector, n_vectors, sigma0,
+                              model_topo<T>(M), M->mn, (double *)A->upd.p, nsplit, A->acc);
+}
+
+template <class T>
+int iter_update_t(somhip_accum *A, T *history_out)
+{
+    if (A->mode != 2) {
+        somhip_set_error("vector-count update without vector-count accumulation");
+        return SOMHIP_ERR_STATE;
+    }
+    somhip_model *M = A->model;
+    M->prepared = false;
+    return somhip_launch_divide(M->ctx->stream, A->acc, M->mn, M->dim, model_w<T>(M), history_out);
+}
+
 }  // namespace
 
 extern "C" {
@@ -542,58 +691,16 @@ int somhip_count_bmus(somhip_model *M, const float *x, const double *rho, int64_
                       double *result)
 {
     int rc = check_model(M);
-    if (rc) return rc;
-    if ((rc = check_prec(precision, arithmetic))) return rc;
-    if (n < 0 || (n > 0 && (!x || !result))) {
-        somhip_set_error("invalid x/result/n");
-        return SOMHIP_ERR_ARG;
-    }
-    if (n == 0) return SOMHIP_OK;
-    somhip_ctx *C = M->ctx;
-    Workspace &W = C->ws;
-    const int nwg = somhip_scatter_nwg(n, M->mn, 1, C->num_cus, SCATTER_TABLE_BUDGET);
-    if ((rc = W.c1.ensure(sizeof(int32_t) * n)) || (rc = W.bounds.ensure(256)) || (rc = W.shifts.ensure(256)) ||
-        (rc = W.table.ensure((size_t)nwg * M->mn * 8)))
-        return rc;
-    int32_t *c1 = (int32_t *)W.c1.p;
-    if ((rc = somhip_launch_bmu(M, x, n, precision, arithmetic, 0, 0, c1, nullptr, nullptr, nullptr))) return rc;
-    {
-        TimerScope ts(C, TK_ABSMAX);
-        if ((rc = somhip_launch_absmax(C->stream, NO_ROWS, 0, 16, rho, 0, rho ? n : 0, (unsigned int *)W.bounds.p,
-                                       C->num_cus)))
-            return rc;
-        if ((rc = somhip_launch_shifts(C->stream, (const unsigned int *)W.bounds.p, 0, 0, rho != nullptr, n,
-                                       (int *)W.shifts.p)))
-            return rc;
-    }
-    TimerScope ts(C, TK_SCATTER);
-    return somhip_launch_scatter(C->stream, c1, NO_ROWS, 0, rho, 0, n, (const int *)W.shifts.p, M->mn, 1, nwg,
-                                 (unsigned long long *)W.table.p, result, C->lds_per_block);
+    if (rc || (rc = check_prec(precision, arithmetic))) return rc;
+    return count_bmus_t(M, x, rho, n, precision, arithmetic, result);
 }
 
 int somhip_compute_errors(somhip_model *M, const float *x, const double *rho, int64_t n, int precision,
                           int arithmetic, double *errors)
 {
     int rc = check_model(M);
-    if (rc) return rc;
-    if ((rc = check_prec(precision, arithmetic))) return rc;
-    if (n < 0 || (n > 0 && (!x || !errors))) {
-        somhip_set_error("invalid x/errors/n");
-        return SOMHIP_ERR_ARG;
-    }
-    if (n == 0) return SOMHIP_OK;
-    somhip_ctx *C = M->ctx;
-    Workspace &W = C->ws;
-    const int nparts = somhip_errors_nparts(n);
-    if ((rc = W.c1.ensure(sizeof(int32_t) * n)) || (rc = W.c2.ensure(sizeof(int32_t) * n)) ||
-        (rc = W.d1.ensure(sizeof(double) * n)) || (rc = W.part.ensure(sizeof(double) * 3 * nparts)))
-        return rc;
-    int32_t *c1 = (int32_t *)W.c1.p, *c2 = (int32_t *)W.c2.p;
-    double *d1 = (double *)W.d1.p;
-    if ((rc = somhip_launch_bmu(M, x, n, precision, arithmetic, 1, -1, c1, c2, d1, nullptr))) return rc;
-    const double penalty = M->topology == SOMHIP_TOPOLOGY_SQUARE ? 2.0 : 1.0;
-    TimerScope ts(C, TK_ERRORS);
-    return somhip_launch_errors(C->stream, c1, c2, d1, rho, n, M->topo, penalty, (double *)W.part.p, nparts, errors);
+    if (rc || (rc = check_prec(precision, arithmetic))) return rc;
+    return compute_errors_t(M, x, rho, n, precision, arithmetic, errors);
 }
 
 int somhip_accum_create(somhip_model *M, somhip_accum **out)
@@ -705,80 +812,26 @@ int somhip_batch_epoch_accumulate(somhip_accum *A, const float *x, const double 
 int somhip_batch_epoch_update(somhip_accum *A, int64_t cur_epoch, int64_t n_epochs, float sigma0, float *history_out)
 {
     if (!A) return SOMHIP_ERR_ARG;
-    if (A->mode != 1) {
-        somhip_set_error("epoch update without epoch accumulation");
-        return SOMHIP_ERR_STATE;
-    }
-    if (n_epochs <= 0 || cur_epoch < 0) {
-        somhip_set_error("invalid epoch counters");
-        return SOMHIP_ERR_ARG;
-    }
-    somhip_model *M = A->model;
-    int rc = check_model(M);
+    int rc = check_model(A->model);
     if (rc) return rc;
-    const double sigma = epoch_sigma(sigma0, cur_epoch, n_epochs);
-    if (M->separable && somhip_update_separable_fits(M->m, M->n, M->dim)) {
-        if ((rc = A->upd.ensure(sizeof(double) * 2 * (size_t)M->mn * (M->dim + 1)))) return rc;
-        TimerScope ts(M->ctx, TK_UPDATE);
-        rc = somhip_launch_update_separable(M->ctx->stream, A->acc, M->topo, M->m, M->n,
-                                            M->topology == SOMHIP_TOPOLOGY_HEXAGONAL ? 1 : 0, M->dim, sigma,
-                                            (double *)A->upd.p, M->w, history_out);
-        M->prepared = false;
-        return rc;
-    }
-    const int nsplit = somhip_update_nsplit(M->mn);
-    if ((rc = A->upd.ensure(sizeof(double) * (size_t)nsplit * M->mn * (M->dim + 1)))) return rc;
-    TimerScope ts(M->ctx, TK_UPDATE);
-    rc = somhip_launch_update(M->ctx->stream, A->acc, M->topo, M->mn, M->dim, sigma, (double *)A->upd.p, nsplit, M->w,
-                              history_out);
-    M->prepared = false;
-    return rc;
+    return epoch_update_t(A, cur_epoch, n_epochs, (double)sigma0, history_out);
 }
 
 int somhip_batch_iter_accumulate(somhip_accum *A, const float *x, const double *rho, int64_t n, int64_t cur_vector,
                                  int64_t n_vectors, float sigma0, int precision, int arithmetic)
 {
     if (!A) return SOMHIP_ERR_ARG;
-    somhip_model *M = A->model;
-    int rc = check_model(M);
+    int rc = check_model(A->model);
     if (rc || (rc = check_prec(precision, arithmetic))) return rc;
-    if (A->mode == 1) {
-        somhip_set_error("accumulator holds epoch sums; reset it first");
-        return SOMHIP_ERR_STATE;
-    }
-    if (n < 0 || (n > 0 && !x) || n_vectors <= 0 || cur_vector < 0) {
-        somhip_set_error("invalid x/n/vector counters");
-        return SOMHIP_ERR_ARG;
-    }
-    A->mode = 2;
-    if (n == 0) return SOMHIP_OK;
-    somhip_ctx *C = M->ctx;
-    Workspace &W = A->ws;
-    const int nsplit = somhip_iter_nsplit(n, M->mn);
-    if ((rc = W.c1.ensure(sizeof(int32_t) * n)) ||
-        (rc = A->upd.ensure(sizeof(double) * (size_t)nsplit * M->mn * (M->dim + 1))))
-        return rc;
-    int32_t *c1 = (int32_t *)W.c1.p;
-    W.c1_x = nullptr;  // c1 no longer holds the epoch path's BMUs
-    if ((rc = somhip_launch_bmu(M, x, n, precision, arithmetic, 0, -1, c1, nullptr, nullptr, nullptr))) return rc;
-    TimerScope ts(C, TK_ITER);
-    return somhip_launch_iter(C->stream, c1, x, M->dim, M->dpad, rho, n, cur_vector, n_vectors, sigma0, M->topo, M->mn,
-                              (double *)A->upd.p, nsplit, A->acc);
+    return iter_accumulate_t(A, x, rho, n, cur_vector, n_vectors, sigma0, precision, arithmetic);
 }
 
 int somhip_batch_iter_update(somhip_accum *A, float *history_out)
 {
     if (!A) return SOMHIP_ERR_ARG;
-    if (A->mode != 2) {
-        somhip_set_error("vector-count update without vector-count accumulation");
-        return SOMHIP_ERR_STATE;
-    }
-    somhip_model *M = A->model;
-    int rc = check_model(M);
+    int rc = check_model(A->model);
     if (rc) return rc;
-    rc = somhip_launch_divide(M->ctx->stream, A->acc, M->mn, M->dim, M->w, history_out);
-    M->prepared = false;
-    return rc;
+    return iter_update_t(A, history_out);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -854,54 +907,14 @@ int somhip_count_bmus_f64(somhip_model *M, const double *x, const double *rho, i
 {
     int rc = check_model(M, true);
     if (rc) return rc;
-    if (n < 0 || (n > 0 && (!x || !result))) {
-        somhip_set_error("invalid x/result/n");
-        return SOMHIP_ERR_ARG;
-    }
-    if (n == 0) return SOMHIP_OK;
-    somhip_ctx *C = M->ctx;
-    Workspace &W = C->ws;
-    const int nwg = somhip_scatter_nwg(n, M->mn, 1, C->num_cus, SCATTER_TABLE_BUDGET);
-    if ((rc = W.c1.ensure(sizeof(int32_t) * n)) || (rc = W.bounds.ensure(256)) || (rc = W.shifts.ensure(256)) ||
-        (rc = W.table.ensure((size_t)nwg * M->mn * 8)))
-        return rc;
-    int32_t *c1 = (int32_t *)W.c1.p;
-    if ((rc = somhip_launch_bmu_f64(M, x, n, 0, 0, c1, nullptr, nullptr))) return rc;
-    {
-        TimerScope ts(C, TK_ABSMAX);
-        if ((rc = somhip_launch_absmax(C->stream, NO_ROWS, 0, 16, rho, 0, rho ? n : 0,
-                                       (unsigned int *)W.bounds.p, C->num_cus)) ||
-            (rc = somhip_launch_shifts(C->stream, (const unsigned int *)W.bounds.p, 0, 0, rho != nullptr, n,
-                                       (int *)W.shifts.p)))
-            return rc;
-    }
-    TimerScope ts(C, TK_SCATTER);
-    return somhip_launch_scatter(C->stream, c1, NO_ROWS, 0, rho, 0, n, (const int *)W.shifts.p, M->mn,
-                                 1, nwg, (unsigned long long *)W.table.p, result, C->lds_per_block);
+    return count_bmus_t(M, x, rho, n, SOMHIP_PREC_EXACT, SOMHIP_RERANK_FP64, result);
 }
 
 int somhip_compute_errors_f64(somhip_model *M, const double *x, const double *rho, int64_t n, double *errors)
 {
     int rc = check_model(M, true);
     if (rc) return rc;
-    if (n < 0 || (n > 0 && (!x || !errors))) {
-        somhip_set_error("invalid x/errors/n");
-        return SOMHIP_ERR_ARG;
-    }
-    if (n == 0) return SOMHIP_OK;
-    somhip_ctx *C = M->ctx;
-    Workspace &W = C->ws;
-    const int nparts = somhip_errors_nparts(n);
-    if ((rc = W.c1.ensure(sizeof(int32_t) * n)) || (rc = W.c2.ensure(sizeof(int32_t) * n)) ||
-        (rc = W.d1.ensure(sizeof(double) * n)) || (rc = W.part.ensure(sizeof(double) * 3 * nparts)))
-        return rc;
-    int32_t *c1 = (int32_t *)W.c1.p, *c2 = (int32_t *)W.c2.p;
-    double *d1 = (double *)W.d1.p;
-    if ((rc = somhip_launch_bmu_f64(M, x, n, 1, -1, c1, c2, d1))) return rc;
-    const double penalty = M->topology == SOMHIP_TOPOLOGY_SQUARE ? 2.0 : 1.0;
-    TimerScope ts(C, TK_ERRORS);
-    return somhip_launch_errors(C->stream, c1, c2, d1, rho, n, (const double *)M->topo64, penalty,
-                                (double *)W.part.p, nparts, errors);
+    return compute_errors_t(M, x, rho, n, SOMHIP_PREC_EXACT, SOMHIP_RERANK_FP64, errors);
 }
 
 int somhip_batch_epoch_accumulate_f64(somhip_accum *A, const double *x, const double *rho, int64_t n, double *errors)
@@ -961,72 +974,26 @@ int somhip_batch_epoch_update_f64(somhip_accum *A, int64_t cur_epoch, int64_t n_
                                   double *history_out)
 {
     if (!A) return SOMHIP_ERR_ARG;
-    somhip_model *M = A->model;
-    int rc = check_model(M, true);
+    int rc = check_model(A->model, true);
     if (rc) return rc;
-    if (A->mode != 1) {
-        somhip_set_error("epoch update without epoch accumulation");
-        return SOMHIP_ERR_STATE;
-    }
-    if (n_epochs <= 0 || cur_epoch < 0) {
-        somhip_set_error("invalid epoch counters");
-        return SOMHIP_ERR_ARG;
-    }
-    const double sigma = epoch_sigma(sigma0, cur_epoch, n_epochs);
-    TimerScope ts(M->ctx, TK_UPDATE);
-    if (M->separable && somhip_update_separable_fits(M->m, M->n, M->dim)) {
-        if ((rc = A->upd.ensure(sizeof(double) * 2 * (size_t)M->mn * (M->dim + 1)))) return rc;
-        return somhip_launch_update_separable(M->ctx->stream, A->acc, (const double *)M->topo64, M->m, M->n,
-                                              M->topology == SOMHIP_TOPOLOGY_HEXAGONAL ? 1 : 0, M->dim, sigma,
-                                              (double *)A->upd.p, M->w64, history_out);
-    }
-    const int nsplit = somhip_update_nsplit(M->mn);
-    if ((rc = A->upd.ensure(sizeof(double) * (size_t)nsplit * M->mn * (M->dim + 1)))) return rc;
-    return somhip_launch_update(M->ctx->stream, A->acc, (const double *)M->topo64, M->mn, M->dim, sigma,
-                                (double *)A->upd.p, nsplit, M->w64, history_out);
+    return epoch_update_t(A, cur_epoch, n_epochs, sigma0, history_out);
 }
 
 int somhip_batch_iter_accumulate_f64(somhip_accum *A, const double *x, const double *rho, int64_t n,
                                      int64_t cur_vector, int64_t n_vectors, double sigma0)
 {
     if (!A) return SOMHIP_ERR_ARG;
-    somhip_model *M = A->model;
-    int rc = check_model(M, true);
+    int rc = check_model(A->model, true);
     if (rc) return rc;
-    if (A->mode == 1) {
-        somhip_set_error("accumulator holds epoch sums; reset it first");
-        return SOMHIP_ERR_STATE;
-    }
-    if (n < 0 || (n > 0 && !x) || n_vectors <= 0 || cur_vector < 0) {
-        somhip_set_error("invalid x/n/vector counters");
-        return SOMHIP_ERR_ARG;
-    }
-    A->mode = 2;
-    if (n == 0) return SOMHIP_OK;
-    somhip_ctx *C = M->ctx;
-    Workspace &W = A->ws;
-    const int nsplit = somhip_iter_nsplit(n, M->mn);
-    if ((rc = W.c1.ensure(sizeof(int32_t) * n)) ||
-        (rc = A->upd.ensure(sizeof(double) * (size_t)nsplit * M->mn * (M->dim + 1))))
-        return rc;
-    int32_t *c1 = (int32_t *)W.c1.p;
-    if ((rc = somhip_launch_bmu_f64(M, x, n, 0, -1, c1, nullptr, nullptr))) return rc;
-    TimerScope ts(C, TK_ITER);
-    return somhip_launch_iter(C->stream, c1, x, M->dim, M->dpad, rho, n, cur_vector, n_vectors, sigma0,
-                              (const double *)M->topo64, M->mn, (double *)A->upd.p, nsplit, A->acc);
+    return iter_accumulate_t(A, x, rho, n, cur_vector, n_vectors, sigma0, SOMHIP_PREC_EXACT, SOMHIP_RERANK_FP64);
 }
 
 int somhip_batch_iter_update_f64(somhip_accum *A, double *history_out)
 {
     if (!A) return SOMHIP_ERR_ARG;
-    somhip_model *M = A->model;
-    int rc = check_model(M, true);
+    int rc = check_model(A->model, true);
     if (rc) return rc;
-    if (A->mode != 2) {
-        somhip_set_error("vector-count update without vector-count accumulation");
-        return SOMHIP_ERR_STATE;
-    }
-    return somhip_launch_divide(M->ctx->stream, A->acc, M->mn, M->dim, M->w64, history_out);
+    return iter_update_t(A, history_out);
 }
 
 }  // extern "C"
