@@ -288,6 +288,35 @@ def test_one_step_multi_slice_scatter_vs_fp64(shape):
     assert rel.max() <= 1e-6, rel.max()
 
 
+def test_sorted_scatter_hot_neurons():
+    """The sorted scatter (maps of 8 or more slices) with neuron runs far longer than one thread's share
+    of entries: 70% of the rows sit at three neurons (one run spans hundreds of threads, so many threads
+    flush into one row), the rest spread over the map; one weighted epoch against the FP64 oracle, and
+    the same epoch from chunks of unequal size."""
+    m, n, dim, N = 100, 100, 32, 400_000
+    print("sorted scatter hot neurons 100x100/D32", flush=True)
+    rng = np.random.default_rng(81)
+    W0 = (rng.standard_normal((m * n, dim)) * 0.5).astype(np.float32)
+    X = rng.standard_normal((N, dim)).astype(np.float32)
+    hot = rng.random(N) < 0.7
+    X[hot] = W0[[17, 4242, 9999]][rng.integers(0, 3, hot.sum())] + \
+        (rng.standard_normal((hot.sum(), dim)) * 1e-3).astype(np.float32)
+    rho = rng.random(N) * 2.0
+    rho[::89] = 0.0
+    sigma = R.sigma_at(50.0, 0, 1)
+    c1, _, _ = R.top2_omp(W0, X, FP64)
+    assert np.bincount(c1, minlength=m * n).max() > 0.2 * N
+    Wf, _, _, _ = R.factorised_epoch_fp64(W0, R.topography(m, n), X, rho.astype(np.float32).astype(np.float64),
+                                          sigma, bmu=c1, out_dtype=np.float64)
+    gen = lambda a: (lambda: lambda: (a[i:j] for i, j in ((0, 150_000), (150_000, 390_000), (390_000, N))))  # noqa: E731
+    for data, weights in ((X, rho), (gen(X), gen(rho))):
+        som = dam.SOM_Batch(m, n, dim, arithmetic="fp64", sigma=50.0)
+        som._weights[...] = W0
+        som.train(data, weights, n_epochs=1, use_best_epoch=False)
+        rel = np.abs(som.weights - Wf) / np.maximum(np.abs(Wf), 1e-3)
+        assert rel.max() <= 1e-6, rel.max()
+
+
 @pytest.mark.parametrize("shape", [(64, 64, 16, "hexagonal"), (32, 40, 32, "square"), (24, 24, 48, "hexagonal")])
 def test_trained_map_bf16_equals_exact(shape):
     """On a map with trained structure (near-equal neighbour distances, the hard case for the BF16
