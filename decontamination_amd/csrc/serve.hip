@@ -48,6 +48,10 @@ constexpr size_t ACT_TABLE_BUDGET = (size_t)32 << 20;
 // the copy stream at a time, taken round-robin from the requests being uploaded.
 constexpr size_t UPLOAD_PIECE = (size_t)16 << 20;
 constexpr int MAX_INFLIGHT = 3;
+// Tile launches of one dimension class queued on the compute stream at a time. While one runs, newly
+// uploaded tiles gather into the next launch instead of queueing as many small launches behind it (a
+// 16 MB piece of D = 32 rows is only 256 tiles, which leaves half of the workgroup slots empty).
+constexpr int MAX_LAUNCHES_INFLIGHT = 1;
 
 inline size_t align_up(size_t v) { return (v + ALIGN - 1) / ALIGN * ALIGN; }
 
@@ -150,7 +154,8 @@ struct somhip_server {
     BmuTile *staging = nullptr;  // pinned host, STAGING_SLOTS * MAX_TILES
     hipEvent_t staging_ev[STAGING_SLOTS] = {};
     int staging_next = 0;
-    std::vector<hipEvent_t> ev_pool;  // completed piece events for reuse
+    std::vector<hipEvent_t> ev_pool;  // completed piece and launch events for reuse
+    std::deque<hipEvent_t> launch_evs[2];  // per dimension class: events after the queued tile launches
     int inflight = 0;                 // pieces queued and not yet seen complete
     size_t up_cursor = 0;             // round-robin position among the requests being uploaded
 
@@ -234,6 +239,15 @@ int somhip_server::launch_tiles(int dc, const std::vector<BmuTile> &tl)
     SOM_HIP_TRY(hipMemcpyAsync(G.tiles, st, sizeof(BmuTile) * BUCKETS[b], hipMemcpyHostToDevice, compute));
     SOM_HIP_TRY(hipEventRecord(staging_ev[slot], compute));
     SOM_HIP_TRY(hipGraphLaunch(G.exec, compute));
+    hipEvent_t e = nullptr;
+    if (!ev_pool.empty()) {
+        e = ev_pool.back();
+        ev_pool.pop_back();
+    } else {
+        SOM_HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    }
+    launch_evs[dc].push_back(e);  // owned by the queue from here, even if the record fails
+    SOM_HIP_TRY(hipEventRecord(e, compute));
     launches++;
     tiles_launched += nt;
     return SOMHIP_OK;
@@ -380,6 +394,18 @@ void somhip_server::loop()
                 else if (R->state == READY && R->next_row >= R->n) done_rows.push_back(R.get());
             }
             if (ready.empty() && done_rows.empty()) continue;
+            // launches of this class that have completed free their slots
+            while (!launch_evs[dc].empty()) {
+                const hipError_t q = hipEventQuery(launch_evs[dc].front());
+                if (q == hipErrorNotReady) break;
+                if (q != hipSuccess) {
+                    fail(std::string("launch event: ") + hipGetErrorString(q));
+                    return;
+                }
+                ev_pool.push_back(launch_evs[dc].front());
+                launch_evs[dc].pop_front();
+            }
+            if ((int)launch_evs[dc].size() >= MAX_LAUNCHES_INFLIGHT) ready.clear();  // they wait for a slot
             std::vector<BmuTile> tl;
             tl.reserve(MAX_TILES);
             std::vector<Request *> finished;
@@ -420,7 +446,7 @@ void somhip_server::loop()
                     return;
                 }
             }
-            progress = true;
+            if (!tl.empty() || !finished.empty()) progress = true;
         }
         // completions
         for (size_t i = 0; i < active.size();) {
@@ -520,6 +546,8 @@ int somhip_server_destroy(somhip_server *S)
         if (kv.second->ev_done) (void)hipEventDestroy(kv.second->ev_done);
     }
     for (auto e : S->ev_pool) (void)hipEventDestroy(e);
+    for (auto &q : S->launch_evs)
+        for (auto e : q) (void)hipEventDestroy(e);
     for (int dc = 0; dc < 2; dc++)
         for (int b = 0; b < NBUCKETS; b++) {
             GraphSlot &G = S->graphs[dc][b];
