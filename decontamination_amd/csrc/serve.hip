@@ -471,9 +471,14 @@ void somhip_server::loop()
             i++;
         }
         if (!progress) {
+            if (!active.empty()) {
+                // work in flight: poll again at once (a timed wait sleeps tens of microseconds past its
+                // timeout, which small requests pay several times over)
+                std::this_thread::yield();
+                continue;
+            }
             std::unique_lock<std::mutex> g(mu);
-            if (pending.empty() || active.size() > 0)
-                cv_work.wait_for(g, std::chrono::microseconds(active.empty() ? 2000 : 20));
+            if (pending.empty()) cv_work.wait_for(g, std::chrono::microseconds(2000));
         }
     }
 }
