@@ -442,10 +442,10 @@ int count_bmus_t(somhip_model *M, const T *x, const double *rho, int64_t n, int 
     if ((rc = search(M, x, n, precision, arithmetic, 0, 0, c1, nullptr, nullptr))) return rc;
     {
         TimerScope ts(C, TK_ABSMAX);
-        if ((rc = somhip_launch_absmax(C->stream, NO_ROWS, 0, 16, rho, 0, rho ? n : 0, (unsigned int *)W.bounds.p,
+        if ((rc = somhip_launch_absmax(C->stream, NO_ROWS, 0, 16, rho, 0, rho ? n : 0, (unsigned long long *)W.bounds.p,
                                        C->num_cus)))
             return rc;
-        if ((rc = somhip_launch_shifts(C->stream, (const unsigned int *)W.bounds.p, 0, 0, rho != nullptr, n,
+        if ((rc = somhip_launch_shifts(C->stream, (const unsigned long long *)W.bounds.p, 0, 0, rho != nullptr, n,
                                        (int *)W.shifts.p)))
             return rc;
     }
@@ -798,10 +798,10 @@ int somhip_batch_epoch_accumulate(somhip_accum *A, const float *x, const double 
     {
         TimerScope ts(C, TK_ABSMAX);
         if (rho &&
-            (rc = somhip_launch_absmax(C->stream, NO_ROWS, 0, 16, rho, 1, n, (unsigned int *)W.bounds.p, C->num_cus)))
+            (rc = somhip_launch_absmax(C->stream, NO_ROWS, 0, 16, rho, 1, n, (unsigned long long *)W.bounds.p, C->num_cus)))
             return rc;
         if ((rc = somhip_launch_shifts_parts(C->stream, (const float *)W.xmax.p, xrows, M->dpad, M->dim,
-                                             rho ? (const unsigned int *)W.bounds.p : nullptr, n, (int *)W.shifts.p)))
+                                             rho ? (const unsigned long long *)W.bounds.p : nullptr, n, (int *)W.shifts.p)))
             return rc;
     }
     TimerScope ts(C, TK_SCATTER);
@@ -940,7 +940,8 @@ int somhip_batch_epoch_accumulate_f64(somhip_accum *A, const double *x, const do
     Workspace &W = A->ws;
     if ((rc = W.c1.ensure(sizeof(int32_t) * n)) || (rc = W.c2.ensure(sizeof(int32_t) * n)) ||
         (rc = W.d1.ensure(sizeof(double) * n)) || (rc = W.part.ensure(sizeof(double) * 3 * nparts)) ||
-        (rc = W.bounds.ensure(512)) || (rc = W.shifts.ensure(512)) ||
+        (rc = W.bounds.ensure(sizeof(unsigned long long) * (M->dim + 1))) ||
+        (rc = W.shifts.ensure(sizeof(int) * (M->dim + 1))) ||
         (rc = W.table.ensure((size_t)nwg * M->mn * nv * 8)) || (rc = W.bucket.ensure(sizeof(unsigned int) * (size_t)n)))
         return rc;
     int32_t *c1 = (int32_t *)W.c1.p, *c2 = (int32_t *)W.c2.p;
@@ -959,9 +960,9 @@ int somhip_batch_epoch_accumulate_f64(somhip_accum *A, const double *x, const do
     {
         // the density stays FP64 (the reference casts it to dtype); scales from max |x_k| and max rho
         TimerScope ts(C, TK_ABSMAX);
-        if ((rc = somhip_launch_absmax(C->stream, x, M->dim, M->dpad, rho, 0, n, (unsigned int *)W.bounds.p,
+        if ((rc = somhip_launch_absmax(C->stream, x, M->dim, M->dpad, rho, 0, n, (unsigned long long *)W.bounds.p,
                                        C->num_cus)) ||
-            (rc = somhip_launch_shifts(C->stream, (const unsigned int *)W.bounds.p, M->dim, 1, rho != nullptr, n,
+            (rc = somhip_launch_shifts(C->stream, (const unsigned long long *)W.bounds.p, M->dim, 1, rho != nullptr, n,
                                        (int *)W.shifts.p)))
             return rc;
     }

@@ -71,8 +71,8 @@ extern "C" int somhip_gndm(somhip_ctx *C, const int32_t *winners, const double *
     // the two activation maps (the reference's _count_bmus_kernel rule: rows with rho > 0 add rho)
     for (int pass = 0; pass < 2 && n > 0; pass++) {
         const double *r = pass == 0 ? rho : nullptr;
-        if ((rc = somhip_launch_absmax(st, NO_ROWS, 0, 16, r, 0, r ? n : 0, (unsigned int *)W.bounds.p, C->num_cus)) ||
-            (rc = somhip_launch_shifts(st, (const unsigned int *)W.bounds.p, 0, 0, r != nullptr, n, (int *)W.shifts.p)) ||
+        if ((rc = somhip_launch_absmax(st, NO_ROWS, 0, 16, r, 0, r ? n : 0, (unsigned long long *)W.bounds.p, C->num_cus)) ||
+            (rc = somhip_launch_shifts(st, (const unsigned long long *)W.bounds.p, 0, 0, r != nullptr, n, (int *)W.shifts.p)) ||
             (rc = somhip_launch_scatter(st, winners, NO_ROWS, 0, r, 0, n, (const int *)W.shifts.p, mn, 1, nwg,
                                         (unsigned long long *)W.table.p, pass == 0 ? cat64 : foot64,
                                         C->lds_per_block)))

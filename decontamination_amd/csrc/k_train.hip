@@ -17,21 +17,16 @@ constexpr size_t LDS_SCATTER_MAX = 160 * 1024;  // neuron slice per scatter work
 constexpr int SCATTER_NT = 1024;                // threads per scatter workgroup
 
 // ------------------------------------------------------------------------------------------------
-// Column-wise max |v| over finite values; out[k] for k < dim is max|x_k|, out[dim] is max rho (as
-// float after the optional float rounding). Stored as float bits with atomicMax (values >= 0); FP64
-// maxima are rounded up to a float, so the stored value stays a bound.
+// Column-wise max |v| over finite values; out[k] for k < dim is max|x_k|, out[dim] is max rho (after
+// the optional float rounding). Stored as FP64 bits with atomicMax (values >= 0 order as their bits),
+// so every finite maximum of float32 or float64 data is held exactly.
 // ------------------------------------------------------------------------------------------------
-__device__ __forceinline__ float float_bound(float v) { return v; }
-__device__ __forceinline__ float float_bound(double v)
-{
-    float f = (float)v;
-    if ((double)f < v) f = nextafterf(f, __builtin_inff());
-    return f;
-}
+__device__ __forceinline__ unsigned long long bound_bits(double v) { return __builtin_bit_cast(unsigned long long, v); }
+__device__ __forceinline__ double bound_value(unsigned long long b) { return __builtin_bit_cast(double, b); }
 
 template <int DP, class T>
 __global__ __launch_bounds__(256) void absmax_kernel(const T *__restrict__ x, int dim, const double *__restrict__ rho,
-                                                    int rho_f32, int64_t n, unsigned int *__restrict__ out)
+                                                    int rho_f32, int64_t n, unsigned long long *__restrict__ out)
 {
     T mx[DP];
 #pragma unroll
@@ -53,42 +48,50 @@ __global__ __launch_bounds__(256) void absmax_kernel(const T *__restrict__ x, in
             if (r > 0.0 && r < 1e300) mr = fmax(mr, r);
         }
     }
-    __shared__ unsigned int red[DP + 1];
-    if (threadIdx.x <= DP) red[threadIdx.x] = 0u;
+    __shared__ unsigned long long red[DP + 1];
+    if (threadIdx.x <= DP) red[threadIdx.x] = 0ull;
     __syncthreads();
     if (x) {
 #pragma unroll
         for (int k = 0; k < DP; k++)
-            if (k < dim) atomicMax(&red[k], __builtin_bit_cast(unsigned int, float_bound(mx[k])));
+            if (k < dim) atomicMax(&red[k], bound_bits((double)mx[k]));
     }
-    // rho bound rounded up to a float so the scale stays safe
-    float mrf = (float)mr;
-    if ((double)mrf < mr) mrf = nextafterf(mrf, __builtin_inff());
-    atomicMax(&red[DP], __builtin_bit_cast(unsigned int, mrf));
+    atomicMax(&red[DP], bound_bits(mr));
     __syncthreads();
     if ((int)threadIdx.x < dim && x) atomicMax(&out[threadIdx.x], red[threadIdx.x]);
     if (threadIdx.x == 0) atomicMax(&out[dim], red[DP]);
 }
 
-// shift_k = floor(61 - log2(B_k * n)): every |rho x| * 2^shift <= 2^61 / n, so the int64 sum of n
-// terms (each rounded once) stays below 2^62. Column dim uses the rho bound (1 when rho is absent).
-__global__ void shifts_kernel(const unsigned int *__restrict__ bounds, int dim, int has_x, int has_rho, int64_t n,
-                              int *__restrict__ shifts)
+// shift = floor(61 - log2(bx * rb * n)) for a column bound bx and a rho bound rb: every |rho x| * 2^shift
+// <= 2^61 / n, so the int64 sum of n terms (each rounded once) stays below 2^62. Where the product
+// overflows (float64 data), the exponents are added instead, a bound at most 4 times looser.
+__device__ int scale_shift(double bx, double rb, int64_t n)
+{
+    if (!(bx > 0.0) || !(rb > 0.0)) return 0;
+    const double nn = (double)(n > 0 ? n : 1);
+    const double t = bx * rb * nn;
+    int e;
+    if (t < __builtin_inf()) {
+        frexp(t, &e);  // t < 2^e
+    } else {
+        int ex, er, en;
+        frexp(bx, &ex);
+        frexp(rb, &er);
+        frexp(nn, &en);
+        e = ex + er + en;
+    }
+    return min(1000, max(-1000, 61 - e));
+}
+
+// Column dim uses the rho bound alone (1 when rho is absent).
+__global__ void shifts_kernel(const unsigned long long *__restrict__ bounds, int dim, int has_x, int has_rho,
+                              int64_t n, int *__restrict__ shifts)
 {
     const int k = threadIdx.x;
     if (k > dim) return;
-    const double rb = has_rho ? (double)__builtin_bit_cast(float, bounds[dim]) : 1.0;
-    double B = (k == dim) ? rb : (has_x ? (double)__builtin_bit_cast(float, bounds[k]) * rb : 0.0);
-    int s = 0;
-    if (B > 0.0) {
-        const double t = B * (double)(n > 0 ? n : 1);
-        int e;
-        frexp(t, &e);  // t < 2^e
-        s = 61 - e;
-        if (s > 1000) s = 1000;
-        if (s < -1000) s = -1000;
-    }
-    shifts[k] = s;
+    const double rb = has_rho ? bound_value(bounds[dim]) : 1.0;
+    const double bx = (k == dim) ? 1.0 : (has_x ? bound_value(bounds[k]) : 0.0);
+    shifts[k] = scale_shift(bx, rb, n);
 }
 
 // parts[b][k] = max over rows r = b, b + RB, b + 2RB, ... of parts[r][k] (b < RB); the other rows are
@@ -113,9 +116,9 @@ __global__ __launch_bounds__(256) void rowmax_fold_kernel(float *__restrict__ pa
 }
 
 // Same scales from the per-wave max |x_k| rows written by the BMU kernels (rows x dpad floats) and an
-// optional max rho (float bits, from absmax_kernel on rho); column dim is the rho column.
+// optional max rho (FP64 bits, from absmax_kernel on rho); column dim is the rho column.
 __global__ __launch_bounds__(1024) void shifts_parts_kernel(const float *__restrict__ parts, int64_t rows, int dpad,
-                                                           int dim, const unsigned int *__restrict__ rho_bound,
+                                                           int dim, const unsigned long long *__restrict__ rho_bound,
                                                            int64_t n, int *__restrict__ shifts)
 {
     __shared__ float red[1024];
@@ -128,24 +131,14 @@ __global__ __launch_bounds__(1024) void shifts_parts_kernel(const float *__restr
     red[threadIdx.x] = m;
     __syncthreads();
     if ((int)threadIdx.x <= dim) {
-        const double rb = rho_bound ? (double)__builtin_bit_cast(float, rho_bound[0]) : 1.0;
-        double B;
-        if ((int)threadIdx.x == dim) {
-            B = rb;
-        } else {
+        const double rb = rho_bound ? bound_value(rho_bound[0]) : 1.0;
+        double bx = 1.0;
+        if ((int)threadIdx.x < dim) {
             float mk = 0.f;
             for (int j = threadIdx.x; j < rs * dpad; j += dpad) mk = fmaxf(mk, red[j]);
-            B = (double)mk * rb;
+            bx = (double)mk;
         }
-        int s = 0;
-        if (B > 0.0) {
-            int e;
-            frexp(B * (double)(n > 0 ? n : 1), &e);
-            s = 61 - e;
-            if (s > 1000) s = 1000;
-            if (s < -1000) s = -1000;
-        }
-        shifts[threadIdx.x] = s;
+        shifts[threadIdx.x] = scale_shift(bx, rb, n);
     }
 }
 
@@ -765,9 +758,9 @@ __global__ __launch_bounds__(256) void errors_final_kernel(const double *__restr
 // ------------------------------------------------------------------------------------------------
 template <class T>
 int somhip_launch_absmax(hipStream_t st, const T *x, int dim, int dpad, const double *rho, int rho_f32, int64_t n,
-                         unsigned int *bounds, int num_cus)
+                         unsigned long long *bounds, int num_cus)
 {
-    SOM_HIP_TRY(hipMemsetAsync(bounds, 0, sizeof(unsigned int) * (dim + 1), st));
+    SOM_HIP_TRY(hipMemsetAsync(bounds, 0, sizeof(unsigned long long) * (dim + 1), st));
     if (n <= 0) return SOMHIP_OK;
     const int64_t blocks = std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, (int64_t)num_cus * 8));
     dim3 g((unsigned)blocks), b(256);
@@ -781,7 +774,7 @@ int somhip_launch_absmax(hipStream_t st, const T *x, int dim, int dpad, const do
     return SOMHIP_OK;
 }
 
-int somhip_launch_shifts(hipStream_t st, const unsigned int *bounds, int dim, int has_x, int has_rho, int64_t n,
+int somhip_launch_shifts(hipStream_t st, const unsigned long long *bounds, int dim, int has_x, int has_rho, int64_t n,
                          int *shifts)
 {
     hipLaunchKernelGGL(shifts_kernel, dim3(1), dim3(128), 0, st, bounds, dim, has_x, has_rho, n, shifts);
@@ -790,7 +783,7 @@ int somhip_launch_shifts(hipStream_t st, const unsigned int *bounds, int dim, in
 }
 
 int somhip_launch_shifts_parts(hipStream_t st, const float *parts, int64_t rows, int dpad, int dim,
-                               const unsigned int *rho_bound, int64_t n, int *shifts)
+                               const unsigned long long *rho_bound, int64_t n, int *shifts)
 {
     // first stage: fold the rows in place into the first RB rows (RB x dpad maxima), then one block
     constexpr int RB = 2048;
@@ -1013,7 +1006,7 @@ int somhip_launch_errors(hipStream_t st, const int32_t *c1, const int32_t *c2, c
 // FP32 (float32 SOMs) and FP64 (float64 SOMs) forms of the templated launchers
 #define SOM_TRAIN_LAUNCHERS(T)                                                                                  \
     template int somhip_launch_absmax<T>(hipStream_t, const T *, int, int, const double *, int, int64_t,        \
-                                         unsigned int *, int);                                                  \
+                                         unsigned long long *, int);                                            \
     template int somhip_launch_scatter<T>(hipStream_t, const int32_t *, const T *, int, const double *, int,    \
                                           int64_t, const int *, int, int, int, unsigned long long *, double *,  \
                                           size_t, unsigned int *);                                              \

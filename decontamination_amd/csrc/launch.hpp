@@ -12,11 +12,11 @@ constexpr size_t SCATTER_TABLE_BUDGET = (size_t)1 << 30;
 constexpr const float *NO_ROWS = nullptr;
 template <class T>
 int somhip_launch_absmax(hipStream_t st, const T *x, int dim, int dpad, const double *rho, int rho_f32, int64_t n,
-                         unsigned int *bounds, int num_cus);
-int somhip_launch_shifts(hipStream_t st, const unsigned int *bounds, int dim, int has_x, int has_rho, int64_t n,
+                         unsigned long long *bounds, int num_cus);
+int somhip_launch_shifts(hipStream_t st, const unsigned long long *bounds, int dim, int has_x, int has_rho, int64_t n,
                          int *shifts);
 int somhip_launch_shifts_parts(hipStream_t st, const float *parts, int64_t rows, int dpad, int dim,
-                               const unsigned int *rho_bound, int64_t n, int *shifts);
+                               const unsigned long long *rho_bound, int64_t n, int *shifts);
 int somhip_scatter_nwg(int64_t n, int mn, int nv, int num_cus, size_t table_budget);
 // bucket (nullable): n uint32 of workspace; with it, maps that need several neuron slices use the
 // bucketed kernel (each slice pass reads only its own samples' entries)

@@ -120,7 +120,7 @@ struct Request {
     int32_t *c1 = nullptr;
     double *act = nullptr;
     unsigned long long *table = nullptr;
-    unsigned int *bounds = nullptr;
+    unsigned long long *bounds = nullptr;
     int *shifts = nullptr;
     int nwg = 1;
     int64_t next_row = 0;     // first row not yet handed to a launch
@@ -346,7 +346,7 @@ void somhip_server::loop()
                 if (R->rho_host) R->rho = (double *)take(sizeof(double) * R->n);
                 if (R->kind == SOMHIP_SERVE_ACTIVATION) {
                     R->act = (double *)take(sizeof(double) * M->mn);
-                    R->bounds = (unsigned int *)take(512);
+                    R->bounds = (unsigned long long *)take(512);
                     R->shifts = (int *)take(512);
                     R->table = (unsigned long long *)take(sizeof(unsigned long long) * (size_t)R->nwg * M->mn);
                 }

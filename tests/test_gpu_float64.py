@@ -119,6 +119,29 @@ def test_float64_training_vs_oracle(topology):
     assert _rel(sv.weights, want) <= 1e-10, _rel(sv.weights, want)
 
 
+def test_float64_values_beyond_float32_range():
+    """Data, weights and densities above the float32 range (3.4e38) in a float64 SOM: the fixed-point
+    scales come from FP64 bounds, so the activation map and training follow the oracle."""
+    m, n, dim, N, E, sigma = 6, 7, 5, 3000, 2, 2.0
+    print("float64 values beyond the float32 range", flush=True)
+    W0, X = _case(m, n, dim, N, 13)
+    W0 *= 1e40
+    X *= 1e40
+    rho = np.random.default_rng(14).random(N) * 1e45
+    rho[::11] = 0.0
+    topo = R.topography(m, n, "hexagonal", np.float64)
+    som = dam.SOM_Batch(m, n, dim, dtype=np.float64, sigma=sigma)
+    som._weights[...] = W0
+    assert np.array_equal(som.get_winners(X), R.find_bmus_f64(W0, X))
+    act = som.get_activation_map(X, rho)
+    assert np.allclose(act.reshape(-1), R.activation_map_f64(W0, X, rho), rtol=1e-12, atol=0)
+    som.train(X, rho, n_epochs=E, use_best_epoch=False)
+    W = W0.copy()
+    for e in range(E):
+        W = R.epoch_f64(W, topo, X, rho, sigma, e, E)
+        assert _rel(som._history[e], W) <= 1e-10, (e, _rel(som._history[e], W))
+
+
 def test_float64_dense_update_on_arbitrary_topography():
     """A topography that is not the closed-form grid takes the dense H.S update."""
     m, n, dim, N = 7, 8, 5, 3000
