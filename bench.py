@@ -786,6 +786,8 @@ def main():
         if r3[head_name].get("roofline"):
             r3[head_name]["roofline"]["mfma_busy"] = profile_figure("som_epoch_100x100_d32_counters.md", "mfma_busy",
                                                                     "bmu_bf16_kernel")
+            tr3 = profile_figure("som_epoch_100x100_d32_hbm_traffic.md", "traffic_bytes_per_launch", "bmu_bf16_kernel")
+            r3[head_name]["roofline"]["traffic"] = None if tr3 is None else int(tr3)
         out["c3"] = {"workload": f"batch SOM epoch, 100x100 hexagonal, dim 32, N={X3.shape[0]} (HEALPix nside 2048, "
                                  "16 lognormal + 14 Gaussian + 2 angular), sigma0=50, timed epochs of the 10-epoch "
                                  "schedule from init_rand(0), X resident in HBM",
