@@ -739,14 +739,23 @@ __device__ __forceinline__ void bmu_bf16_body(const BmuModelDev &Md, const float
             mycnt = min(cnt[g], CAPL);
             ovf = lost[g];
         }
-        if (valid_s && !ovf) {
+        // The two lanes of a sample share its candidates: the pair's lists, the khalf 0 lane's first,
+        // are split in two halves, one per lane, so a wave runs as many candidate distances as its most
+        // loaded pair's mean, not its longest list. Top-2 insertion is order-independent.
+        const int pcnt = xchg32i(mycnt);
+        const bool povf = xchg32i(ovf ? 1 : 0) != 0;
+        if (valid_s && !ovf && !povf) {
             float xr[DP];
 #pragma unroll
             for (int k = 0; k < DP; k++) xr[k] = k < dim ? x[s * dim + k] : 0.f;
+            const int nlo = khalf == 0 ? mycnt : pcnt;  // entries of the khalf 0 lane's list
+            const int half = (mycnt + pcnt + 1) / 2;
+            const int eb = khalf == 0 ? 0 : half, ee = khalf == 0 ? half : mycnt + pcnt;
             // candidate rows come from the slot-ordered copy of W (no index indirection before the
             // load); the next candidate's row is loaded while this one is evaluated
             auto load_row = [&](int e, float (&wr)[DP], int &j) {
-                const int slot = cw[g][e][lane];
+                const bool lo = e < nlo;
+                const int slot = cw[g][lo ? e : e - nlo][lo ? col : col + 32];
                 j = perm[slot];
                 const auto row = as_f32x4(wslot + (int64_t)slot * DP);
 #pragma unroll
@@ -758,19 +767,19 @@ __device__ __forceinline__ void bmu_bf16_body(const BmuModelDev &Md, const float
             if constexpr (DP <= 16) {
                 float wn[DP];
                 int jn = -1;
-                if (mycnt > 0) load_row(0, wn, jn);
-                for (int e = 0; e < mycnt; e++) {
+                if (eb < ee) load_row(eb, wn, jn);
+                for (int e = eb; e < ee; e++) {
                     float wr[DP];
 #pragma unroll
                     for (int k = 0; k < DP; k++) wr[k] = wn[k];
                     const int j = jn;
-                    if (e + 1 < mycnt) load_row(e + 1, wn, jn);
+                    if (e + 1 < ee) load_row(e + 1, wn, jn);
                     if (j >= 0) tp.push(sqd_mode<MODE>(wr, xr, DP), j);  // zero-padded dims add exact zeros
                 }
             } else {
                 // wider rows: one row in registers at a time, so the rerank does not set the kernel's
                 // register peak (which would halve the waves per SIMD)
-                for (int e = 0; e < mycnt; e++) {
+                for (int e = eb; e < ee; e++) {
                     float wr[DP];
                     int j;
                     load_row(e, wr, j);
@@ -808,7 +817,6 @@ __device__ __forceinline__ void bmu_bf16_body(const BmuModelDev &Md, const float
         const int pi1 = xchg32i(tp.i1), pi2 = xchg32i(tp.i2);
         if (pi1 >= 0) tp.push(pd1, pi1);
         if (pi2 >= 0) tp.push(pd2, pi2);
-        const bool povf = xchg32i(ovf ? 1 : 0) != 0;
         const bool bad = ovf || povf || !(T[g] < __builtin_inff()) || (T[g] != T[g]);
         if (lane < 32 && valid_s) {
             if (bad) {
