@@ -344,6 +344,29 @@ def epoch_f64(W, topo, X, rho, sigma0, e, n_epochs):
     return _divide_f64(*_accumulate_f64(W, topo, X, rho, 1.0 / (2.0 * s * s)))
 
 
+def factorised_epoch_f64(W, topo, X, rho, sigma0, e, n_epochs, bmu):
+    """epoch_f64 in the factorised form num = H.S, den = H.c for given winners bmu (-1: no winner): in epoch
+    mode h depends only on (j, bmu(i)) (som_batch.py:217-228), so the per-sample scatter of
+    _train_step2_xpu (som_batch.py:509-566) sums to H times the per-winner sums. FP64 throughout."""
+    W = np.asarray(W, np.float64)
+    mn, dim = W.shape
+    s = max(float(sigma0) * (1.0 / (1.0 + 2.0 * e / n_epochs)), FLOAT_EPS)
+    r = np.ones(X.shape[0]) if rho is None else np.asarray(rho, np.float64)
+    bmu = np.asarray(bmu)
+    keep = (r > 0.0) & (bmu >= 0)
+    xk = np.asarray(X, np.float64)[keep]
+    S = np.stack([np.bincount(bmu[keep], weights=xk[:, k] * r[keep], minlength=mn) for k in range(dim)], 1)
+    cnt = np.bincount(bmu[keep], weights=r[keep], minlength=mn)
+    t = np.asarray(topo, np.float64)
+    num = np.empty((mn, dim))
+    den = np.empty(mn)
+    for j0 in range(0, mn, 1024):  # H by row blocks
+        H = np.exp(-sqd_f64(t[j0:j0 + 1024, None, :], t[None, :, :]) * (1.0 / (2.0 * s * s)))
+        num[j0:j0 + 1024] = H @ S
+        den[j0:j0 + 1024] = H @ cnt
+    return _divide_f64(num, den)
+
+
 def iter_f64(W, topo, X, rho, sigma0, cur, n_vectors):
     """Vector-count mode of a float64 SOM over X (som_batch.py:154-210, 386-486): sigma_i =
     sigma0 / (1 + 2 (cur + i) / n_vectors) >= 1e-12 per sample, then the division."""

@@ -142,6 +142,29 @@ def test_float64_values_beyond_float32_range():
         assert _rel(som._history[e], W) <= 1e-10, (e, _rel(som._history[e], W))
 
 
+@pytest.mark.parametrize("shape", [(64, 64, 16, "hexagonal"), (100, 100, 16, "hexagonal"), (100, 100, 32, "square")])
+def test_float64_multi_slice_scatter(shape):
+    """float64 maps whose per-neuron table needs several LDS slices: the bucketed scatter over FP64 rows at
+    64x64/D16 and 100x100/D32 and the sorted one at 100x100/D16. One weighted epoch against the
+    factorised FP64 oracle on the winners of the same weights, which are checked against the float64
+    oracle on a subsample."""
+    m, n, dim, topology = shape
+    print(f"float64 multi-slice scatter {m}x{n}/D{dim} {topology}", flush=True)
+    N = 30_000
+    rng = np.random.default_rng(90 + dim + m)
+    W0 = rng.standard_normal((m * n, dim)) / 3.0
+    X = rng.standard_normal((N, dim)) / 2.0
+    rho = rng.random(N) * 2.0
+    rho[::17] = 0.0
+    som = dam.SOM_Batch(m, n, dim, dtype=np.float64, topology=topology)
+    som._weights[...] = W0
+    c1 = som.get_winners(X)
+    assert np.array_equal(c1[:600], R.find_bmus_f64(W0, X[:600]))
+    som.train(X, rho, n_epochs=1, use_best_epoch=False)
+    want = R.factorised_epoch_f64(W0, R.topography(m, n, topology, np.float64), X, rho, max(m, n) / 2.0, 0, 1, c1)
+    assert _rel(som.weights, want) <= 1e-10, _rel(som.weights, want)
+
+
 def test_float64_dense_update_on_arbitrary_topography():
     """A topography that is not the closed-form grid takes the dense H.S update."""
     m, n, dim, N = 7, 8, 5, 3000

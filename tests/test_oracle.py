@@ -368,3 +368,24 @@ def test_float64_oracle_brute_force_and_fp64_topography():
     _, te32 = R.compute_errors(Wn.astype(np.float32), R.topography(2, 4), Xs.astype(np.float32), None, "hexagonal",
                                FP64)
     assert te32 == 0.0
+
+
+@pytest.mark.parametrize("topology", ["square", "hexagonal"])
+def test_float64_factorised_epoch_equals_per_sample_form(topology):
+    """factorised_epoch_f64 (H times the per-winner sums) equals the per-sample scatter of epoch_f64 to
+    FP64 rounding, with and without densities, zero densities, a NaN row (no winner) and the schedule's
+    sigma at a later epoch."""
+    rng = np.random.default_rng(33)
+    m, n, dim = 5, 6, 4
+    W = rng.standard_normal((m * n, dim)) / 3.0
+    X = rng.standard_normal((400, dim)) / 5.0
+    X[7] = np.nan
+    rho = rng.random(400) * 2.0
+    rho[::9] = 0.0
+    topo = R.topography(m, n, topology, np.float64)
+    c1, _, _ = R.top2_f64(W, X)
+    assert c1[7] == -1
+    for r, e in ((None, 0), (rho, 0), (rho, 3)):
+        want = R.epoch_f64(W, topo, X, r, 2.5, e, 5)
+        got = R.factorised_epoch_f64(W, topo, X, r, 2.5, e, 5, c1)
+        assert np.allclose(got, want, rtol=1e-12, atol=1e-15), np.abs(got - want).max()
