@@ -161,6 +161,30 @@ __device__ __forceinline__ void load_sample(XP x, int64_t n, int dim, int64_t s,
     }
 }
 
+// The same for rows of exactly KC * 16 floats at a 16-byte aligned base: two 16-byte loads per 8 dims,
+// no per-dimension tests.
+template <int KC, class XP, class CP>
+__device__ __forceinline__ void load_sample_rows(XP x, int64_t n, int64_t s, int lane, CP centre, SampleLoad<KC> &L)
+{
+    const int hb = 8 * (lane >> 5);
+#pragma unroll
+    for (int kc = 0; kc < KC; kc++) {
+        f32x4 a = {0.f, 0.f, 0.f, 0.f}, b = {0.f, 0.f, 0.f, 0.f};
+        if (s < n) {
+            const auto row = as_f32x4(x + s * (KC * 16) + kc * 16 + hb);
+            a = row[0];
+            b = row[1];
+        }
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+            const float v = e < 4 ? a[e] : b[e - 4];
+            L.xc[kc][e] = __fsub_rn(v, centre[kc * 16 + hb + e]);
+            const float av = fabsf(v);
+            L.ax[kc][e] = av < __builtin_inff() ? av : 0.f;
+        }
+    }
+}
+
 // Per-wave max of |x| per dimension: lanes l and l^32 hold different dims, the 32 lanes of a half
 // hold the same dims for different samples. Writes DP floats for this wave (plain stores).
 template <int KC>
@@ -392,6 +416,8 @@ __device__ __forceinline__ void bmu_bf16_body(const BmuModelDev &Md, const float
     const auto d1 = aptr<GC>(d1_out);
     const auto fb_list = aptr<GC>(fb_list_out);
     const int nblk = Md.nblk, dim = Md.dim;
+    // sample rows of exactly DP floats at a 16-byte aligned base are read with vector loads
+    const bool rows_vec = dim == DP && ((uintptr_t)x_in & 15) == 0;
     const int tid = threadIdx.x;
     const int lane = tid & (WAVE - 1);
     const int wave = tid / WAVE;
@@ -472,7 +498,10 @@ __device__ __forceinline__ void bmu_bf16_body(const BmuModelDev &Md, const float
     for (int g = 0; g < G; g++) {
         const int64_t s = tile_row0 + rl[g];
         SampleLoad<KC> L;
-        load_sample<KC>(x, n, dim, s, lane, centre, L);
+        if (rows_vec)
+            load_sample_rows<KC>(x, n, s, lane, centre, L);
+        else
+            load_sample<KC>(x, n, dim, s, lane, centre, L);
 #pragma unroll
         for (int kc = 0; kc < KC; kc++)
 #pragma unroll
@@ -747,7 +776,19 @@ __device__ __forceinline__ void bmu_bf16_body(const BmuModelDev &Md, const float
         if (valid_s && !ovf && !povf) {
             float xr[DP];
 #pragma unroll
-            for (int k = 0; k < DP; k++) xr[k] = k < dim ? x[s * dim + k] : 0.f;
+            for (int k = 0; k < DP; k++) xr[k] = 0.f;
+            if (rows_vec) {
+                const auto row = as_f32x4(x + s * DP);
+#pragma unroll
+                for (int q = 0; q < DP / 4; q++) {
+                    const f32x4 t = row[q];
+                    xr[4 * q] = t[0]; xr[4 * q + 1] = t[1]; xr[4 * q + 2] = t[2]; xr[4 * q + 3] = t[3];
+                }
+            } else {
+#pragma unroll
+                for (int k = 0; k < DP; k++)
+                    if (k < dim) xr[k] = x[s * dim + k];
+            }
             const int nlo = khalf == 0 ? mycnt : pcnt;  // entries of the khalf 0 lane's list
             const int half = (mycnt + pcnt + 1) / 2;
             const int eb = khalf == 0 ? 0 : half, ee = khalf == 0 ? half : mycnt + pcnt;
