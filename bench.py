@@ -731,7 +731,8 @@ def main():
     res = wl["res"]
     h = res[head_name]
     default_workload = m == 64 and args.nside == 1024 and dim == 16
-    if h["roofline"] is not None and default_workload:
+    # the committed counter summaries profile the FP64-rerank training pass
+    if h["roofline"] is not None and default_workload and args.arithmetic == "fp64":
         tr = profile_figure("som_epoch_64x64_d16_hbm_traffic.md", "traffic_bytes_per_launch", "bmu_bf16_kernel")
         h["roofline"]["traffic"] = None if tr is None else int(tr)
         h["roofline"]["mfma_busy"] = profile_figure("som_epoch_64x64_d16_counters.md", "mfma_busy",
@@ -783,7 +784,7 @@ def main():
                             (1 << 18, 1 << 16, 1 << 15), "c3")
         w3["x_dev"].free()
         r3 = w3["res"]
-        if r3[head_name].get("roofline"):
+        if r3[head_name].get("roofline") and args.arithmetic == "fp64":
             r3[head_name]["roofline"]["mfma_busy"] = profile_figure("som_epoch_100x100_d32_counters.md", "mfma_busy",
                                                                     "bmu_bf16_kernel")
             tr3 = profile_figure("som_epoch_100x100_d32_hbm_traffic.md", "traffic_bytes_per_launch", "bmu_bf16_kernel")
